@@ -1,0 +1,163 @@
+"""Benchmark of the plain relaxing-TVD split MHD step on one MI355X.
+
+Workload (default): one 1024^3 periodic FP32 box, seeded random-phase IC with
+velocity amplitude 0.3 and vector-potential amplitude 0.05, CFL 0.7.  A step
+is one full x,y,z,z,y,x split step (calcfl, 6 fluid sweeps, 6 CT field
+sweeps, 4 transposes) replayed from a captured HIP graph; state, t and dt
+stay in HBM.  Prints one JSON line:
+
+  value      cell-updates/s = N * steps / wall time of the timed steps
+  roofline   the dominant kernel's algorithmic bytes per launch / its mean
+             duration (HIP events on the launch stream) vs 8 TB/s
+  cpu_baseline  the oracle restatement (bitwise equal to the reference C
+             path) timed on this host's cores on a bounded 128^3 sample
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+PEAK_HBM_GBS = 8000.0
+# compulsory bytes per cell per call at each operator's C-call boundary (FP32)
+ALGO_BYTES = {"fluidx": 52, "advectbyzx": 32, "transpose12": 64, "transpose13": 64, "calcfl": 32}
+CALLS_PER_STEP = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13": 2, "calcfl": 1}
+METRIC = "cell-updates/s & % HBM roofline per 3D step"
+
+
+def kernel_times(box, reps: int):
+    """Mean device time (s) of each operator's launch, by HIP events recorded
+    on the stream the kernels are launched on."""
+    import torch
+    from heterogeneous_mhd_amd import ops
+    nx, ny, nz = box.nx, box.ny, box.nz
+    u, b, u2, b2 = box.u, box.b, box.u2, box.b2
+    dt = box.dt_tensor()
+    calls = {
+        "fluidx": lambda: ops.fluidx(u2, b, nx, ny, nz, dt, u, b),
+        "advectbyzx": lambda: ops.advectbyzx(u, b2, nx, ny, nz, dt, u, b),
+        "transpose12": lambda: ops.transpose12(u2, b2, u, b, nx, ny, nz),
+        "transpose13": lambda: ops.transpose13(u2, b2, u, b, nx, ny, nz),
+        "calcfl": lambda: ops.calcfl_device(u, b, nx, ny, nz),
+    }
+    out = {}
+    with torch.cuda.stream(box.stream):
+        for name, f in calls.items():
+            if name == "advectbyzx":
+                b2.copy_(b)  # advectbyzx reads main b(1) from its output buffer
+            f()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+            for r in range(reps):
+                ev[2 * r].record()
+                f()
+                ev[2 * r + 1].record()
+            torch.cuda.synchronize()
+            out[name] = float(np.mean([ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps)])) / 1e3
+    return out
+
+
+def cpu_baseline(budget_s: float = 15.0):
+    """Oracle (FP32, bitwise equal to the reference C path) on host cores."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from heterogeneous_mhd_amd import solver
+    from oracle import Oracle
+    threads = min(16, os.cpu_count() or 1)
+    n = 128
+    u, b = solver.ic_host(n, n, n, solver.ICSpec(seed=1, amp_v=0.3, amp_a=0.05))
+    o = Oracle("f32", threads=threads)
+    o.run(u, b, n, n, n, 1)  # warm-up step
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        u, b, _, _ = o.run(u, b, n, n, n, 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 6:
+            break
+    return {"value": n ** 3 * steps / el, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle mhd_ref.c FP32 (bitwise equal to the reference C path), OpenMP over "
+                      f"pencils, {n}^3 box, {steps} full steps after 1 warm-up, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--amp-v", type=float, default=0.3)
+    ap.add_argument("--amp-a", type=float, default=0.05)
+    ap.add_argument("--cfl", type=float, default=0.7)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the step would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from heterogeneous_mhd_amd import solver
+    n = args.n
+    N = n ** 3
+    print(f"[bench] {n}^3 box, allocating state", file=sys.stderr, flush=True)
+    box = solver.Box(n, n, n, cfl=args.cfl)
+    box.init(solver.ICSpec(seed=1, amp_v=args.amp_v, amp_a=args.amp_a))
+    graph = not args.no_graph
+    print("[bench] IC ready, warm-up", file=sys.stderr, flush=True)
+    box.run(args.warmup, graph=graph)
+    box.sync()
+    torch.cuda.synchronize()
+    print("[bench] timing", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    box.run(args.steps, graph=graph)
+    box.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_sim, dt = box.time()
+    ms_per_step = 1e3 * wall / args.steps
+    value = N * args.steps / wall
+    print(f"[bench] {ms_per_step:.2f} ms/step, t={t_sim:.4f} dt={dt:.4g}", file=sys.stderr, flush=True)
+
+    kt = kernel_times(box, args.kernel_reps)
+    per_step = {k: kt[k] * CALLS_PER_STEP[k] for k in kt}
+    dom = max(per_step, key=per_step.get)
+    achieved = ALGO_BYTES[dom] * N / kt[dom] / 1e9
+    step_bytes = sum(ALGO_BYTES[k] * CALLS_PER_STEP[k] for k in ALGO_BYTES) * N
+    diag = box.diagnostics()
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random-phase Fourier IC generated on the GPU)",
+        "config": {"workload": f"plain relaxing-TVD split MHD step, {n}^3 periodic FP32 box, "
+                               f"v amp {args.amp_v}, A amp {args.amp_a}, CFL {args.cfl}",
+                   "nx": n, "ny": n, "nz": n, "graph": graph},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "algo_bytes_per_cell": ALGO_BYTES[dom], "mean_launch_s": kt[dom]},
+        "step_roofline": {"algo_bytes_per_cell_step": step_bytes / N,
+                          "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
+                          "frac": step_bytes / (wall / args.steps) / 1e9 / PEAK_HBM_GBS},
+        "kernel_ms": {k: 1e3 * v for k, v in kt.items()},
+        "divb_max": diag["divb_max"],
+    }
+    del box
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline()
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,70 @@
+"""Build recipe for libmhd_hip.so (gfx950) and the CPU checker libraries.
+
+``python -m heterogeneous_mhd_amd.build`` compiles every HIP source under
+``csrc/`` with hipcc into ``heterogeneous_mhd_amd/libmhd_hip.so`` (in-tree, so
+the built library travels with the source tree), then runs ``make`` in
+``oracle/`` for the CPU checkers.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libmhd_hip.so"
+
+SOURCES = ["mhd_fluid.hip", "mhd_advect.hip", "mhd_transpose.hip", "mhd_cfl.hip", "mhd_ic.hip",
+           "mhd_capi.hip"]
+
+# -ffp-contract=off: no FMA contraction, so float/double rounding follows
+# the reference's C promotion rules exactly (bitwise parity).
+# -packed-fp32-ops off: packed FP32 instructions are not used.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def build_hip(verbose: bool = False) -> Path:
+    srcs = [str(CSRC / s) for s in SOURCES]
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), *srcs, "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(with_reference: bool | None = None, verbose: bool = False) -> None:
+    targets = ["all"]
+    refdir = Path("/root/reference/C")
+    if with_reference is None:
+        with_reference = refdir.is_dir()
+    if with_reference:
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), *targets], check=True,
+                   stdout=None if verbose else subprocess.DEVNULL)
+
+
+def build_all(verbose: bool = False) -> None:
+    build_hip(verbose)
+    build_oracle(verbose=verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-v" in sys.argv)

@@ -1,0 +1,232 @@
+// mhd_advect.hip -- advectbyzx: constrained-transport advection of the two
+// transverse face fields along the sweep axis.
+//
+// Reference: advectbyzx C/advectbyzx.c:14-23 (A1, A2, B1, B2 in order),
+// advectbyzxA1 C/advectbyzxA1.c:18-63, A2 C/advectbyzxA2.c:9-28,
+// B1 C/advectbyzxB1.c:19-62, B2 C/advectbyzxB2.c:9-28, tvdb C/tvdb.c:10-91.
+//
+// The reference runs four full passes and scatters the edge flux of pencil
+// (j,k) into b1 of pencil (j-1,k) (A2) and (j,k-1) (B2) through a global tmp
+// array, so that per cell
+//   b1 <- (((b1 - FA(j,k)) + FA(j+1,k)) - FB(j,k)) + FB(j,k+1)     (at i-1)
+// Here one workgroup owns pencil (j,k) and evaluates the four tvdb lines it
+// needs -- A at (j,k) and (j+1,k), B at (j,k) and (j,k+1) -- itself, so
+// there is no tmp array, no scatter, no atomics and a single pass; b1 is
+// written once in the reference's addition order.  The output b must not
+// alias update_b (neighbour pencils read update_b while this one writes b);
+// b_main is read only at the cell being written, so it may alias b.
+#include "mhd_internal.h"
+
+namespace mhd {
+namespace {
+
+constexpr int kLines = 4;  // A(j), A(j+1), B(k), B(k+1)
+
+template <int BLOCK, int M>
+__global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out,
+                                                       const float *b_main,
+                                                       const float *__restrict__ uu,
+                                                       const float *__restrict__ bu, int nx,
+                                                       int ny, int nz, int tp,
+                                                       const float *__restrict__ dtp, float dtv)
+{
+    extern __shared__ float lds[];
+    const int pb = BLOCK / tp;
+    const int lp = threadIdx.x / tp;
+    const int lt = threadIdx.x - lp * tp;
+    const long npen = (long)ny * nz;
+    const long pen = (long)blockIdx.x * pb + lp;
+    const bool live = pen < npen;
+    const int j = (int)(pen % ny), k = (int)(pen / ny);
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
+    const int km = k == 0 ? nz - 1 : k - 1, kp = k + 1 == nz ? 0 : k + 1;
+    const float dt = dtp ? *dtp : dtv;
+    const float z = 0.0f, h = 0.5f;
+
+    // three stencil arrays of kLines*nx floats per pencil, line-major
+    float *X = lds + (size_t)lp * 3 * kLines * nx;
+    float *Y = X + kLines * nx;
+    float *Z = Y + kLines * nx;
+
+    // pencil (row) offsets in cells
+    const size_t row = (size_t)nx;
+    const size_t o_c = ((size_t)k * ny + j) * row;
+    const size_t o_jm = ((size_t)k * ny + jm) * row, o_jp = ((size_t)k * ny + jp) * row;
+    const size_t o_km = ((size_t)km * ny + j) * row, o_kp = ((size_t)kp * ny + j) * row;
+    // line t: face velocity from pencils (lo, hi); advected field comp / pencil
+    const size_t lo[kLines] = {o_jm, o_c, o_km, o_c};
+    const size_t hi[kLines] = {o_c, o_jp, o_c, o_kp};
+    const size_t fb[kLines] = {o_c, o_jp, o_c, o_kp};
+    const int comp[kLines] = {1, 1, 2, 2};
+
+    float bl[M][kLines];  // the advected face field, per line
+    float b1m[M];         // main b1 of this pencil
+    // ---- raw face velocity vx = (m1(lo)+m1(hi))/(rho(lo)+rho(hi)), A1 :34 / B1 :34
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            float cu[2];
+            cu[0] = uu[(o_c + i) * 5 + 0];
+            cu[1] = uu[(o_c + i) * 5 + 1];
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                const size_t a = lo[t] + i, bidx = hi[t] + i;
+                const float rl = (lo[t] == o_c) ? cu[0] : uu[a * 5 + 0];
+                const float ml = (lo[t] == o_c) ? cu[1] : uu[a * 5 + 1];
+                const float rh = (hi[t] == o_c) ? cu[0] : uu[bidx * 5 + 0];
+                const float mh = (hi[t] == o_c) ? cu[1] : uu[bidx * 5 + 1];
+                X[t * nx + i] = (ml + mh) / (rl + rh);
+                bl[r][t] = bu[(fb[t] + i) * 3 + comp[t]];
+            }
+            b1m[r] = b_main[(o_c + i) * 3 + 0];
+        }
+    }
+    __syncthreads();
+    // ---- smoothing vx = (vx(i-1)+vx(i+1)+2vx)/4 in double, A1 :37-41
+    float vg[M][kLines];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                const float s = X[t * nx + im] + X[t * nx + ip];
+                vg[r][t] = (float)(((double)s + 2.0 * (double)X[t * nx + i]) / 4.0);
+                Y[t * nx + i] = vg[r][t];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- tvdb, C/tvdb.c:26-46: vh, and b*vg staged for the upwind pick
+    float vh[M][kLines];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int ip = i + 1 == nx ? 0 : i + 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                vh[r][t] = h * vg[r][t] + h * Y[t * nx + ip] + z;
+                X[t * nx + i] = bl[r][t] * vg[r][t];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int ip = i + 1 == nx ? 0 : i + 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++)
+                Z[t * nx + i] = vh[r][t] > 0.0f ? X[t * nx + i] : X[t * nx + ip];  // flux1
+        }
+    }
+    __syncthreads();
+    // ---- half step b1 = b - (flux1 - flux1(i-1))*dt/2, C/tvdb.c:48-52
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int im = i == 0 ? nx - 1 : i - 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                const float d = (Z[t * nx + i] - Z[t * nx + im]) * dt;
+                X[t * nx + i] = (float)((double)bl[r][t] - (double)d / 2.0);
+            }
+        }
+    }
+    __syncthreads();
+    // ---- limited flux, C/tvdb.c:54-78
+    float fx[M][kLines];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int ip = i + 1 == nx ? 0 : i + 1, ipp = ip + 1 == nx ? 0 : ip + 1;
+            const int im = i == 0 ? nx - 1 : i - 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                const float *bb = X + t * nx, *vv = Y + t * nx;
+                float w, wp, wm;
+                if (vh[r][t] > 0.0f) {
+                    w = vv[i] * bb[i];
+                    wp = (float)((double)(vv[ip] * bb[ip] - w) / 2.0);
+                    wm = (float)((double)(w - vv[im] * bb[im]) / 2.0);
+                } else {
+                    w = vv[ip] * bb[ip];
+                    wp = (float)((double)(w - vv[ipp] * bb[ipp]) / 2.0);
+                    wm = (float)((double)(vv[i] * bb[i] - w) / 2.0);
+                }
+                float dw = 0.0f;
+                if (wm * wp > 0.0f) dw = (float)(2.0 * (double)wm * (double)wp / (double)(wm + wp));
+                fx[r][t] = (w + dw) * dt;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+#pragma unroll
+            for (int t = 0; t < kLines; t++) Z[t * nx + i] = fx[r][t];
+        }
+    }
+    __syncthreads();
+    // ---- b -= flux - flux(i-1) (C/tvdb.c:80-88); b1 gets the four edge
+    // fluxes F(i-1) in the order A1, A2, B1, B2 apply them.
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int im = i == 0 ? nx - 1 : i - 1;
+            float fm[kLines];
+#pragma unroll
+            for (int t = 0; t < kLines; t++) fm[t] = Z[t * nx + im];
+            const float b2n = bl[r][0] - (fx[r][0] - fm[0]);
+            const float b3n = bl[r][2] - (fx[r][2] - fm[2]);
+            const float b1n = (((b1m[r] - fm[0]) + fm[1]) - fm[2]) + fm[3];
+            float *o = b_out + (o_c + i) * 3;
+            o[0] = b1n;
+            o[1] = b2n;
+            o[2] = b3n;
+        }
+    }
+}
+
+template <int BLOCK, int M>
+int launch_cfg(float *b_out, const float *b_main, const float *uu, const float *bu, int nx, int ny,
+               int nz, int tp, const float *dtp, float dtv, hipStream_t s)
+{
+    const int pb = BLOCK / tp;
+    const long npen = (long)ny * nz;
+    const long nblk = (npen + pb - 1) / pb;
+    const size_t lds = sizeof(float) * 3 * kLines * (size_t)nx * pb;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)advect_kernel<BLOCK, M>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipGetLastError();  // clear a stale error so the check below sees this launch
+    hipLaunchKernelGGL((advect_kernel<BLOCK, M>), dim3((unsigned)nblk), dim3(BLOCK), lds, s, b_out,
+                       b_main, uu, bu, nx, ny, nz, tp, dtp, dtv);
+    return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+}  // namespace
+
+int launch_advectbyzx(float *b_out, const float *b_main, const float *uu, const float *bu, int nx,
+                      int ny, int nz, const float *dt_dev, float dt_val, hipStream_t s)
+{
+    if (!b_out || !b_main || !uu || !bu || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
+    if (nx > kMaxPencil) return MHD_ERR_TOO_LONG;
+    if (nx <= 64) return launch_cfg<256, 1>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
+    if (nx <= 128) return launch_cfg<256, 2>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
+    if (nx <= 256) return launch_cfg<256, 4>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
+    if (nx <= 512) return launch_cfg<256, 4>(b_out, b_main, uu, bu, nx, ny, nz, 128, dt_dev, dt_val, s);
+    if (nx <= 1024) return launch_cfg<256, 4>(b_out, b_main, uu, bu, nx, ny, nz, 256, dt_dev, dt_val, s);
+    return launch_cfg<512, 4>(b_out, b_main, uu, bu, nx, ny, nz, 512, dt_dev, dt_val, s);
+}
+
+}  // namespace mhd

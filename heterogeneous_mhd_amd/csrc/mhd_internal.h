@@ -1,0 +1,63 @@
+// mhd_internal.h -- shared declarations of the gfx950 HIP kernels' host-side
+// launchers.  Everything here is internal to libmhd_hip.so; the public C ABI
+// is include/mhd_hip.h.
+//
+// State layout (the reference's Fortran order, C/array_definition.h:12-14):
+//   u[((k*ny + j)*nx + i)*5 + c]  c = rho, m1, m2, m3, E
+//   b[((k*ny + j)*nx + i)*3 + c]  c = b1, b2, b3 (face-staggered: b_d at index
+//                                  i sits on the low face of cell i along d)
+// Each operator sweeps along its own first axis (nx); callers permute the
+// dimensions after transposes exactly as C/mhd.c does.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#define MHD_OK 0
+#define MHD_ERR_ARG -1        // bad dimensions or null pointer
+#define MHD_ERR_SHAPE -2      // transpose13 with nx != nz
+#define MHD_ERR_HIP -3        // a HIP runtime call failed
+#define MHD_ERR_TOO_LONG -4   // pencil longer than the kernel's LDS plan allows
+#define MHD_ERR_NOMEM -5      // device allocation failed
+
+namespace mhd {
+
+// Largest pencil (first-axis length) the sweep kernels accept.
+constexpr int kMaxPencil = 2048;
+
+int launch_fluidx(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
+                  const float *dt_dev, float dt_val, hipStream_t s);
+
+int launch_advectbyzx(float *b_out, const float *b_main, const float *uu, const float *bu, int nx,
+                      int ny, int nz, const float *dt_dev, float dt_val, hipStream_t s);
+
+int launch_transpose12(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
+                       int nz, hipStream_t s);
+
+int launch_transpose13(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
+                       int nz, hipStream_t s);
+
+// CFL reduction workspace: partial records + a completion counter.  The
+// counter must be zero before the first launch; the kernel leaves it zero.
+size_t cfl_workspace_bytes(size_t ncell);
+// Writes inv_c[0] = 1/c (float, the reference's calcfl return value).  When
+// dt_dev is non-null also applies the driver's dt rule on the device:
+//   dt = (float)(cfl * (double)inv_c); if (tf > 0 && dt > (tf - t)/2) dt = (tf - t)/2;
+//   t = t + 2*dt     (t_dev may be null: then no clamp and no t update)
+int launch_calcfl(float *inv_c, const float *u, const float *b, int nx, int ny, int nz, void *ws,
+                  double cfl, float *dt_dev, float *t_dev, float tf, hipStream_t s);
+
+// Seeded initial condition (mhd_ic.hip).
+int ic_mode_count(int kmax);
+int ic_generate_host(float *u, float *b, int nx, int ny, int nz, uint64_t seed, double amp_v,
+                     double amp_a, double rho0, double p0, double gamma, int kmax);
+int ic_generate_device(float *u, float *b, int nx, int ny, int nz, uint64_t seed, double amp_v,
+                       double amp_a, double rho0, double p0, double gamma, int kmax, double *A,
+                       hipStream_t s);
+
+// FP64 diagnostics (mhd_ic.hip).
+size_t diag_workspace_bytes(size_t ncell);
+int diagnostics(const float *u, const float *b, int nx, int ny, int nz, void *ws, double *out,
+                hipStream_t s);
+
+}  // namespace mhd

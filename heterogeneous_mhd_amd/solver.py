@@ -1,0 +1,145 @@
+"""Device-resident driver of the full MHD step, the seeded initial condition
+and the FP64 diagnostics.
+
+``Box`` owns one periodic box on the GPU: the state (u, b) in the reference
+layout, a second pair of buffers for the ping-pong that replaces the
+reference's copy_matrix calls, and a native step plan (``mhd_plan``) that
+keeps t and dt on the device.  ``Box.run(n)`` advances n full steps in the
+sweep order of C/mhd.c:59-124, optionally by replaying a captured HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import MHDError, check, lib
+
+GAMMA = 5.0 / 3.0
+
+
+@dataclass(frozen=True)
+class ICSpec:
+    """Seeded random-phase initial condition (see include/mhd_hip.h)."""
+    seed: int = 1
+    amp_v: float = 0.1
+    amp_a: float = 0.05
+    rho0: float = 1.0
+    p0: float = 0.6
+    gamma: float = GAMMA
+    kmax: int = 4
+
+
+def ic_host(nx: int, ny: int, nz: int, spec: ICSpec = ICSpec()):
+    """Initial condition on the host: returns (u, b) float32 numpy arrays of
+    5*N and 3*N elements.  Bitwise equal to :func:`ic_device`."""
+    u = np.empty(5 * nx * ny * nz, np.float32)
+    b = np.empty(3 * nx * ny * nz, np.float32)
+    check(lib().mhd_ic_generate_host(u.ctypes.data, b.ctypes.data, nx, ny, nz, spec.seed,
+                                     spec.amp_v, spec.amp_a, spec.rho0, spec.p0, spec.gamma,
+                                     spec.kmax), "ic_host")
+    return u, b
+
+
+def ic_device(nx: int, ny: int, nz: int, spec: ICSpec = ICSpec(), device="cuda", u=None, b=None):
+    """Initial condition generated on the GPU into (u, b) (allocated if None)."""
+    n = nx * ny * nz
+    if u is None:
+        u = torch.empty(5 * n, dtype=torch.float32, device=device)
+    if b is None:
+        b = torch.empty(3 * n, dtype=torch.float32, device=device)
+    scratch = torch.empty(3 * n, dtype=torch.float64, device=u.device)
+    check(lib().mhd_ic_generate(u.data_ptr(), b.data_ptr(), nx, ny, nz, spec.seed, spec.amp_v,
+                                spec.amp_a, spec.rho0, spec.p0, spec.gamma, spec.kmax,
+                                scratch.data_ptr(), torch.cuda.current_stream().cuda_stream),
+          "ic_device")
+    del scratch
+    return u, b
+
+
+def diagnostics(u: torch.Tensor, b: torch.Tensor, nx: int, ny: int, nz: int) -> dict:
+    """FP64 conserved sums and div b on the device (deterministic)."""
+    nbytes = lib().mhd_diag_workspace_bytes(nx, ny, nz)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=u.device)
+    out = (C.c_double * 7)()
+    check(lib().mhd_diagnostics(u.data_ptr(), b.data_ptr(), nx, ny, nz, ws.data_ptr(), out,
+                                torch.cuda.current_stream().cuda_stream), "diagnostics")
+    v = list(out)
+    return {"mass": v[0], "mom": (v[1], v[2], v[3]), "energy": v[4], "divb_max": v[5],
+            "divb_rms": v[6]}
+
+
+class Box:
+    """One periodic box advanced on the GPU by the native step driver."""
+
+    def __init__(self, nx: int, ny: int, nz: int, cfl: float = 0.7, tf: float = 0.0,
+                 device="cuda", stream: torch.cuda.Stream | None = None):
+        if ny != nz:
+            raise MHDError("the split step needs ny == nz (transpose13 swaps y and z)")
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = nx * ny * nz
+        self.device = torch.device(device)
+        # a stream of its own (default priority): HIP graphs cannot be
+        # captured on the null stream
+        self.stream = stream if stream is not None else torch.cuda.Stream(self.device)
+        self.u = torch.zeros(5 * self.n, dtype=torch.float32, device=self.device)
+        self.b = torch.zeros(3 * self.n, dtype=torch.float32, device=self.device)
+        self.u2 = torch.empty_like(self.u)
+        self.b2 = torch.empty_like(self.b)
+        h = C.c_void_p()
+        check(lib().mhd_plan_create(C.byref(h), nx, ny, nz, float(cfl), float(tf),
+                                    self.stream.cuda_stream), "plan_create")
+        self._plan = h
+
+    def close(self):
+        if getattr(self, "_plan", None):
+            lib().mhd_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def set_state(self, u, b, t: float = 0.0):
+        self.u.copy_(torch.as_tensor(u).reshape(-1))
+        self.b.copy_(torch.as_tensor(b).reshape(-1))
+        torch.cuda.current_stream(self.device).synchronize()
+        check(lib().mhd_plan_set_time(self._plan, float(t)), "set_time")
+
+    def init(self, spec: ICSpec = ICSpec()):
+        with torch.cuda.stream(self.stream):
+            ic_device(self.nx, self.ny, self.nz, spec, u=self.u, b=self.b)
+        self.stream.synchronize()
+        check(lib().mhd_plan_set_time(self._plan, 0.0), "set_time")
+
+    # -- stepping ----------------------------------------------------------
+    def run(self, nsteps: int, graph: bool = False):
+        check(lib().mhd_plan_run(self._plan, self.u.data_ptr(), self.b.data_ptr(),
+                                 self.u2.data_ptr(), self.b2.data_ptr(), int(nsteps), int(graph)),
+              "plan_run")
+
+    def time(self):
+        t, dt = C.c_float(), C.c_float()
+        check(lib().mhd_plan_get_time(self._plan, C.byref(t), C.byref(dt)), "get_time")
+        return t.value, dt.value
+
+    def dt_tensor(self) -> torch.Tensor:
+        """The current dt as a one-element device tensor (a copy)."""
+        return torch.tensor([self.time()[1]], dtype=torch.float32, device=self.device)
+
+    def sync(self):
+        check(lib().mhd_plan_sync(self._plan), "sync")
+
+    def host_state(self):
+        """(u, b) as host numpy arrays, after the box's queued work finished."""
+        self.sync()
+        return self.u.cpu().numpy(), self.b.cpu().numpy()
+
+    def diagnostics(self) -> dict:
+        with torch.cuda.stream(self.stream):
+            return diagnostics(self.u, self.b, self.nx, self.ny, self.nz)

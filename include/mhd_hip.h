@@ -1,0 +1,162 @@
+/*
+ * mhd_hip.h -- C ABI of libmhd_hip.so, an MI355X (gfx950) implementation of
+ * the relaxing-TVD, dimensionally split, constrained-transport MHD step of
+ * uelipen/Heterogeneous_MHD.
+ *
+ * Part 1 is the reference's own call surface: the prototypes of
+ * C/spu_function.h:1-15 with C linkage and identical argument meaning, so
+ * a C driver written against the reference (C/mhd.c) links against this
+ * library unchanged.  State pointers may be device pointers (hipMalloc, the
+ * fast path) or ordinary host pointers (staged through device memory).
+ * These calls are synchronous, like the reference's.
+ *
+ * Part 2 adds what the reference lacks: status codes, explicit HIP streams,
+ * a device-resident dt, the full step driven on the device (optionally
+ * replayed from a captured HIP graph), the seeded initial condition and the
+ * FP64 diagnostics.
+ *
+ * State layout (C/array_definition.h:12): u[((k*ny+j)*nx+i)*5+c] holds
+ * (rho, m1, m2, m3, E); b[((k*ny+j)*nx+i)*3+c] holds the face-staggered
+ * field, b_d at index i on the low face of cell i along axis d.  FP32.
+ */
+#ifndef MHD_HIP_H
+#define MHD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Part 1: the reference interface (C/spu_function.h:1-15)             */
+/* ------------------------------------------------------------------ */
+
+/* replaces fluidx, C/fluidx.c:19 -- fluid sweep along the first axis:
+ * reads update_u/update_b, writes u.  iter is accepted and unused, as in
+ * the reference. */
+void fluidx(float *u, float *b, int *nx, int *ny, int *nz, float *dt, float *update_u,
+            float *update_b, int iter);
+
+/* replaces copy_matrix, C/copy_matrix.c:9 -- update <- main, all 8 comps. */
+void copy_matrix(float *u, float *b, int *nx, int *ny, int *nz, float *update_u, float *update_b);
+
+/* replaces advectbyzx, C/advectbyzx.c:14 -- CT advection of b2, b3 along
+ * the first axis: reads update_u, update_b(2,3) and main b(1), writes b. */
+void advectbyzx(float *u, float *b, int *nx, int *ny, int *nz, float *dt, float *update_u,
+                float *update_b);
+
+/* replaces transpose12, C/transpose12.c:7 -- x<->y swap of update_* into
+ * u, b (out dims (ny,nx,nz)).  iter is unused. */
+void transpose12(float *u, float *b, float *update_u, float *update_b, int *nx, int *ny, int *nz,
+                 int iter);
+
+/* replaces transpose13, C/transpose13.c:9 -- x<->z swap (out dims
+ * (nz,ny,nx)); needs nx == nz, otherwise prints the reference's message
+ * and writes nothing. */
+void transpose13(float *u, float *b, float *update_u, float *update_b, int *nx, int *ny, int *nz);
+
+/* replaces calcfl, C/calcfl.c:13 -- returns 1/c, c the global maximum of
+ * the fast-speed bound (bit-exact). */
+float calcfl(float *u, float *b, int *nx, int *ny, int *nz);
+
+/* ------------------------------------------------------------------ */
+/* Part 2: extensions                                                  */
+/* ------------------------------------------------------------------ */
+
+#define MHD_OK 0
+#define MHD_ERR_ARG -1       /* bad dimensions or null pointer */
+#define MHD_ERR_SHAPE -2     /* transpose13 with nx != nz; step with ny != nz */
+#define MHD_ERR_HIP -3       /* a HIP runtime call failed */
+#define MHD_ERR_TOO_LONG -4  /* pencil longer than mhd_max_pencil() */
+#define MHD_ERR_NOMEM -5     /* device allocation failed */
+
+typedef void *mhd_stream_t; /* a hipStream_t; NULL is the null stream */
+
+/* Library version string and the longest pencil the sweeps accept. */
+const char *mhd_version(void);
+int mhd_max_pencil(void);
+/* Last HIP error text seen by the library ("" if none). */
+const char *mhd_last_error(void);
+
+/* Asynchronous device-pointer forms of Part 1.  dt_dev, when non-null, is a
+ * device pointer the kernels read dt from (so a captured graph picks up the
+ * current step's dt); otherwise dt is used.  advectbyzx's b must not alias
+ * update_b; b_main may alias b.  Transposes must not alias in and out. */
+int mhd_fluidx_async(float *u, const float *update_u, const float *update_b, int nx, int ny, int nz,
+                     const float *dt_dev, float dt, mhd_stream_t stream);
+int mhd_advectbyzx_async(float *b, const float *b_main, const float *update_u,
+                         const float *update_b, int nx, int ny, int nz, const float *dt_dev,
+                         float dt, mhd_stream_t stream);
+int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
+                          int ny, int nz, mhd_stream_t stream);
+int mhd_transpose13_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
+                          int ny, int nz, mhd_stream_t stream);
+int mhd_copy_matrix_async(const float *u, const float *b, int nx, int ny, int nz, float *update_u,
+                          float *update_b, mhd_stream_t stream);
+
+/* calcfl on the device.  workspace: mhd_calcfl_workspace_bytes() bytes of
+ * device memory whose first 4 bytes are zero before its first use (the
+ * kernel leaves them zero).  Writes inv_c_dev[0] = 1/c.  When dt_dev is
+ * non-null it also applies the driver's rule of C/mhd.c:62-64:
+ * dt = (float)(cfl * 1/c); if t_dev is non-null and tf > 0, dt is clamped to
+ * (tf - t)/2, and t += 2 dt. */
+size_t mhd_calcfl_workspace_bytes(int nx, int ny, int nz);
+int mhd_calcfl_async(float *inv_c_dev, const float *u, const float *b, int nx, int ny, int nz,
+                     void *workspace, double cfl, float *dt_dev, float *t_dev, float tf,
+                     mhd_stream_t stream);
+
+/* ---- full step on the device (the sweep order of C/mhd.c:59-124) ---- */
+typedef struct mhd_plan mhd_plan;
+
+/* A plan for boxes of nx*ny*nz (ny == nz, as transpose13 requires at the
+ * driver's call sites).  cfl is the CFL factor (the reference uses 0.9);
+ * tf > 0 enables the reference's end-time clamp.  Owns the device scalars
+ * t and dt and the CFL workspace. */
+int mhd_plan_create(mhd_plan **plan, int nx, int ny, int nz, double cfl, float tf,
+                    mhd_stream_t stream);
+void mhd_plan_destroy(mhd_plan *plan);
+/* Device pointers of the plan's t and dt (floats). */
+float *mhd_plan_t_dev(mhd_plan *plan);
+float *mhd_plan_dt_dev(mhd_plan *plan);
+/* Sets t (device copy, synchronous). */
+int mhd_plan_set_time(mhd_plan *plan, float t);
+/* Advances (u,b) by nsteps full steps.  (u2,b2) are scratch buffers of the
+ * same sizes.  The state is in (u,b) on return.  use_graph != 0 captures one
+ * step into a HIP graph on first use and replays it.  Asynchronous on the
+ * plan's stream. */
+int mhd_plan_run(mhd_plan *plan, float *u, float *b, float *u2, float *b2, int nsteps, int use_graph);
+/* Copies t and dt to the host (synchronises the plan's stream). */
+int mhd_plan_get_time(mhd_plan *plan, float *t, float *dt);
+/* Synchronises the plan's stream. */
+int mhd_plan_sync(mhd_plan *plan);
+
+/* ---- seeded initial condition ---- */
+/* rho = rho0, p = p0, gamma; velocity and vector potential A are sums of
+ * random-phase Fourier modes with integer wavevector 0 < |m| <= kmax
+ * (one of each +-m pair), per-component RMS amp_v (velocity, in units of
+ * the cell) and amp_a (A, in units where the box's first side is 2*pi);
+ * b = discrete curl of A on the staggered faces (div b = 0 to round-off);
+ * E = p0/(gamma-1) + |m|^2/(2 rho) + |Bc|^2/2 with Bc the cell-centred face
+ * average the fluid sweep uses.  Host and device forms give bitwise equal
+ * arrays.  scratch_dev: 3*nx*ny*nz doubles of device memory (for A). */
+int mhd_ic_mode_count(int kmax);
+int mhd_ic_generate_host(float *u, float *b, int nx, int ny, int nz, uint64_t seed, double amp_v,
+                         double amp_a, double rho0, double p0, double gamma, int kmax);
+int mhd_ic_generate(float *u_dev, float *b_dev, int nx, int ny, int nz, uint64_t seed, double amp_v,
+                    double amp_a, double rho0, double p0, double gamma, int kmax,
+                    double *scratch_dev, mhd_stream_t stream);
+
+/* ---- FP64 diagnostics on the device ---- */
+/* out[0..4]: sums of rho, m1, m2, m3, E in double; out[5] = max |div b|;
+ * out[6] = RMS div b.  workspace: mhd_diag_workspace_bytes() bytes.
+ * Deterministic (fixed reduction order).  Synchronous: out is host memory. */
+size_t mhd_diag_workspace_bytes(int nx, int ny, int nz);
+int mhd_diagnostics(const float *u_dev, const float *b_dev, int nx, int ny, int nz, void *workspace,
+                    double *out, mhd_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHD_HIP_H */

@@ -1,0 +1,253 @@
+"""CPU checks that pin the oracle (oracle/mhd_ref.c) before it is trusted.
+
+* golden vectors written from the reference's own compiled C kernels
+  (tests/golden/make_golden.py): bitwise, FP32 and FP64;
+* where /root/reference is present, a live re-run of the compiled reference;
+* closed forms and invariants chosen so that a dropped term, a wrong sign or
+  index, or a transposed operand fails one of them.
+"""
+import numpy as np
+import pytest
+
+from conftest import OPS_CASES, TRAJ_CASES, bits_equal, load_ops, load_traj, sha
+from oracle import HERE as ORACLE_DIR
+from oracle import Oracle, RefC
+
+O32 = Oracle("f32")
+O64 = Oracle("f64")
+
+
+@pytest.mark.parametrize("case", OPS_CASES)
+def test_operators_match_reference_golden(case):
+    d = load_ops(case)
+    nx, ny, nz = d["dims"]
+    u, b, dt = d["u"], d["b"], d["dt"]
+    assert bits_equal(O32.fluidx(u, b, nx, ny, nz, dt), d["fluidx_u"])
+    assert bits_equal(O32.advectbyzx(u, b, nx, ny, nz, dt), d["advectbyzx_b"])
+    assert O32.calcfl(u, b, nx, ny, nz).tobytes() == d["calcfl"].tobytes()
+    tu, tb = O32.transpose12(u, b, nx, ny, nz)
+    assert sha(tu) == d["digests"]["transpose12_u"] and sha(tb) == d["digests"]["transpose12_b"]
+    if nx == nz:
+        tu, tb = O32.transpose13(u, b, nx, ny, nz)
+        assert sha(tu) == d["digests"]["transpose13_u"] and sha(tb) == d["digests"]["transpose13_b"]
+    u64, b64 = u.astype(np.float64), b.astype(np.float64)
+    dt64 = float(dt)
+    assert sha(O64.fluidx(u64, b64, nx, ny, nz, dt64)) == d["digests"]["f64_fluidx_u"]
+    assert sha(O64.advectbyzx(u64, b64, nx, ny, nz, dt64)) == d["digests"]["f64_advectbyzx_b"]
+    assert float(O64.calcfl(u64, b64, nx, ny, nz)) == d["digests"]["f64_calcfl"]
+
+
+@pytest.mark.parametrize("case", TRAJ_CASES)
+def test_trajectories_match_reference_golden(case):
+    from heterogeneous_mhd_amd import solver
+    tr = load_traj(case)
+    nx, ny, nz = tr["dims"]
+    u, b = solver.ic_host(nx, ny, nz, solver.ICSpec(**tr["spec"]))
+    assert sha(u) == tr["ic_sha_u"] and sha(b) == tr["ic_sha_b"]
+    for prec, O in (("f32", O32), ("f64", O64)):
+        seen = []
+        O.set_threads(4)
+        uf, bf, t, dts = O.run(u, b, nx, ny, nz, tr["nsteps"], cfl=tr["cfl"],
+                               on_step=lambda s, uu, bb, dt: seen.append((sha(uu), sha(bb), float(dt))))
+        O.set_threads(1)
+        for s, ref in enumerate(tr[prec]["steps"]):
+            assert seen[s] == (ref["sha_u"], ref["sha_b"], ref["dt"]), f"{prec} step {s}"
+        assert float(t) == tr[prec]["t"]
+
+
+def test_final_state_fixture_16():
+    from heterogeneous_mhd_amd import solver
+    tr = load_traj("16")
+    z = np.load(ORACLE_DIR.parent / "tests" / "golden" / "traj_16_final.npz", allow_pickle=False)
+    u, b = solver.ic_host(16, 16, 16, solver.ICSpec(**tr["spec"]))
+    uf, bf, _, _ = O32.run(u, b, 16, 16, 16, tr["nsteps"], cfl=tr["cfl"])
+    assert bits_equal(uf, z["u"]) and bits_equal(bf, z["b"])
+
+
+refc_missing = not (ORACLE_DIR / "_ref" / "libmhd_refc_f32.so").exists()
+
+
+@pytest.mark.skipif(refc_missing, reason="oracle/_ref not built (no /root/reference here)")
+@pytest.mark.parametrize("case", ["rough_12x10x10", "7x3x7"])
+def test_oracle_matches_live_reference(case):
+    """Re-runs the compiled reference on fresh seeded inputs (not only the
+    stored ones), including 3 full steps on a non-cubic box."""
+    d = load_ops(case)
+    nx, ny, nz = d["dims"]
+    rng = np.random.default_rng(42)
+    u = d["u"] * (1 + 0.02 * rng.standard_normal(d["u"].shape)).astype(np.float32)
+    b = d["b"] + 0.02 * rng.standard_normal(d["b"].shape).astype(np.float32)
+    u, b = u.astype(np.float32), b.astype(np.float32)
+    for prec in ("f32", "f64"):
+        O, R = Oracle(prec), RefC(prec)
+        uu, bb = u.astype(O.dtype), b.astype(O.dtype)
+        assert bits_equal(O.fluidx(uu, bb, nx, ny, nz, 0.13), R.fluidx(uu, bb, nx, ny, nz, 0.13))
+        assert bits_equal(O.advectbyzx(uu, bb, nx, ny, nz, 0.13), R.advectbyzx(uu, bb, nx, ny, nz, 0.13))
+        assert O.calcfl(uu, bb, nx, ny, nz) == R.calcfl(uu, bb, nx, ny, nz)
+    from heterogeneous_mhd_amd import solver
+    u, b = solver.ic_host(20, 12, 12, solver.ICSpec(seed=8, amp_v=0.3, amp_a=0.1))
+    a = O32.run(u, b, 20, 12, 12, 3)
+    r = RefC("f32").run(u, b, 20, 12, 12, 3)
+    assert bits_equal(a[0], r[0]) and bits_equal(a[1], r[1]) and a[3] == r[3]
+
+
+# --------------------------------------------------------------------------
+# closed forms and invariants
+# --------------------------------------------------------------------------
+
+def _rand(nx, ny, nz, seed=0):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal(5 * nx * ny * nz).astype(np.float32)
+    u[0::5] = 1 + 0.2 * np.abs(u[0::5])
+    u[4::5] = 2 + np.abs(u[4::5])
+    b = (0.3 * rng.standard_normal(3 * nx * ny * nz)).astype(np.float32)
+    return u, b
+
+
+def test_transposes_are_the_closed_form_permutation():
+    nx, ny, nz = 6, 5, 6
+    u, b = _rand(nx, ny, nz)
+    U, B = u.reshape(nz, ny, nx, 5), b.reshape(nz, ny, nx, 3)
+    tu, tb = O32.transpose12(u, b, nx, ny, nz)
+    assert bits_equal(tu.reshape(nz, nx, ny, 5), U.transpose(0, 2, 1, 3)[..., [0, 2, 1, 3, 4]])
+    assert bits_equal(tb.reshape(nz, nx, ny, 3), B.transpose(0, 2, 1, 3)[..., [1, 0, 2]])
+    tu, tb = O32.transpose13(u, b, nx, ny, nz)
+    assert bits_equal(tu.reshape(nx, ny, nz, 5), U.transpose(2, 1, 0, 3)[..., [0, 3, 2, 1, 4]])
+    assert bits_equal(tb.reshape(nx, ny, nz, 3), B.transpose(2, 1, 0, 3)[..., [2, 1, 0]])
+    # involutions
+    uu, bb = O32.transpose12(*O32.transpose12(u, b, nx, ny, nz), ny, nx, nz)
+    assert bits_equal(uu, u) and bits_equal(bb, b)
+    uu, bb = O32.transpose13(*O32.transpose13(u, b, nx, ny, nz), nz, ny, nx)
+    assert bits_equal(uu, u) and bits_equal(bb, b)
+    with pytest.raises(ValueError):
+        O32.transpose13(u[: 5 * 4 * 5 * 6], b[: 3 * 4 * 5 * 6], 4, 5, 6)
+
+
+def test_uniform_state_is_a_fixed_point():
+    """All flux differences vanish exactly on a uniform state: fluidx and
+    advectbyzx return it bit for bit (no signed zeros involved)."""
+    nx, ny, nz = 8, 6, 6
+    q = np.array([1.3, 0.2, -0.1, 0.05, 2.7], np.float32)
+    bb = np.array([0.4, -0.3, 0.2], np.float32)
+    u = np.tile(q, nx * ny * nz)
+    b = np.tile(bb, nx * ny * nz)
+    assert bits_equal(O32.fluidx(u, b, nx, ny, nz, 0.3), u)
+    out = O32.advectbyzx(u, b, nx, ny, nz, 0.3)
+    # b2, b3 unchanged bitwise; b1 receives (b1 - F) + F, equal up to round-off
+    assert bits_equal(out.reshape(-1, 3)[:, 1:], b.reshape(-1, 3)[:, 1:])
+    assert np.allclose(out[0::3], b[0::3], rtol=0, atol=1e-6)
+
+
+def test_zero_velocity_leaves_b_unchanged():
+    nx, ny, nz = 10, 4, 4
+    u, b = _rand(nx, ny, nz, 3)
+    u[1::5] = 0.0
+    assert bits_equal(O32.advectbyzx(u, b, nx, ny, nz, 0.2), b)
+
+
+def test_fluidx_conserves_each_pencil_in_fp64():
+    """Flux-difference form: the pencil sum of each conserved component is
+    unchanged up to round-off (checked in the FP64 oracle)."""
+    nx, ny, nz = 32, 3, 3
+    u, b = _rand(nx, ny, nz, 5)
+    out = O64.fluidx(u.astype(np.float64), b.astype(np.float64), nx, ny, nz, 0.05)
+    s0 = u.astype(np.float64).reshape(nz * ny, nx, 5).sum(1)
+    s1 = out.reshape(nz * ny, nx, 5).sum(1)
+    assert np.abs(s1 - s0).max() < 1e-12 * nx
+
+
+def test_advect_preserves_div_b_in_fp64():
+    nx, ny, nz = 12, 10, 10
+    u, b = _rand(nx, ny, nz, 6)
+    u[1::5] *= 0.3
+    b64 = b.astype(np.float64)
+    out = O64.advectbyzx(u.astype(np.float64), b64, nx, ny, nz, 0.1)
+
+    def div(bb):
+        B = bb.reshape(nz, ny, nx, 3)
+        return ((np.roll(B[..., 0], -1, 2) - B[..., 0]) + (np.roll(B[..., 1], -1, 1) - B[..., 1])
+                + (np.roll(B[..., 2], -1, 0) - B[..., 2]))
+
+    assert np.abs(div(out) - div(b64)).max() < 1e-13
+    assert np.abs(out - b64).max() > 1e-3  # the field did move
+
+
+def _cfl_cell_brute(q, bb):
+    """C/calcfl.c:45-54 written with numpy scalars (float32 storage, float64
+    where the reference promotes)."""
+    f, d = np.float32, np.float64
+    g = f(5.0 / 3.0)
+    rho = q[0]
+    t = [f(abs(d(q[c] / rho))) for c in (1, 2, 3)]
+    v = (t[0] if t[0] > t[2] else t[2]) if t[0] > t[1] else (t[1] if t[1] > t[2] else t[2])
+    msq = q[1] * q[1] + q[2] * q[2] + q[3] * q[3]
+    bsq = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2]
+    ps = f((d(q[4]) - d(msq / rho) / 2.0) * (d(g) - 1.0) + (2.0 - d(g)) * d(bsq) / 2.0)
+    p = f(d(ps) - d(bsq) / 2.0)
+    return f(d(v) + np.sqrt(abs((d(bsq) * 2.0 + d(g * p)) / d(rho))))
+
+
+def test_calcfl_brute_force_and_closed_form():
+    nx, ny, nz = 4, 3, 3
+    u, b = _rand(nx, ny, nz, 7)
+    c = np.float32(0)
+    for s in range(nx * ny * nz):
+        t = _cfl_cell_brute(u[5 * s:5 * s + 5], b[3 * s:3 * s + 3])
+        assert O32.cfl_cell(u[5 * s:5 * s + 5], b[3 * s:3 * s + 3]) == t
+        c = c if c > t else t
+    assert O32.calcfl(u, b, nx, ny, nz) == np.float32(1) / c
+    # uniform state: c = max|v| + sqrt((2 B^2 + gamma p)/rho), p = E - m^2/2rho - B^2/2 with
+    # the reference's pressure (which uses gamma, so p = (gamma-1)(E - m^2/2rho) - ...)
+    rho, m, bb, e = 1.25, np.array([0.5, -0.25, 0.125]), np.array([0.5, 0.25, -0.5]), 3.0
+    q = np.array([rho, *m, e], np.float32)
+    u1 = np.tile(q, 8)
+    b1 = np.tile(bb.astype(np.float32), 8)
+    g = 5.0 / 3.0
+    b2 = (bb ** 2).sum()
+    p = (e - (m ** 2).sum() / rho / 2) * (g - 1) + (2 - g) * b2 / 2 - b2 / 2
+    c_exact = np.abs(m / rho).max() + np.sqrt((2 * b2 + g * p) / rho)
+    assert abs(1.0 / O32.calcfl(u1, b1, 2, 2, 2) - c_exact) < 1e-6 * c_exact
+
+
+def test_calcfl_nan_restarts_running_max():
+    """max_2num(c, NaN) returns NaN and max_2num(NaN, x) returns x: the result
+    is the max of the cells after the last NaN."""
+    nx, ny, nz = 5, 2, 2
+    u, b = _rand(nx, ny, nz, 8)
+    n = nx * ny * nz
+    cells = np.array([_cfl_cell_brute(u[5 * s:5 * s + 5], b[3 * s:3 * s + 3]) for s in range(n)])
+    uu = u.copy()
+    uu[5 * 7] = np.nan
+    assert O32.calcfl(uu, b, nx, ny, nz) == np.float32(1) / cells[8:].max()
+    uu[5 * (n - 1)] = np.nan
+    assert np.isnan(O32.calcfl(uu, b, nx, ny, nz))
+
+
+def test_alfven_wave_kat():
+    """The reference's (dead-code) linear Alfven wave, C/init.c:58-122:
+    rho=1, Bx=1, vy = 0.1 sin(2 pi (i+1)/n), By = -vy.  It travels at v_A = 1
+    and returns to its start after t = n; the FP64 oracle reproduces it to
+    the scheme's truncation error (a tolerance KAT, not a bitwise one)."""
+    n = 32
+    i = np.arange(n)
+    vy = 0.1 * np.sin(2 * np.pi * (i + 1) / n)
+    u = np.zeros((1, 1, n, 5))
+    b = np.zeros((1, 1, n, 3))
+    u[..., 0] = 1.0
+    u[0, 0, :, 2] = vy
+    b[..., 0] = 1.0
+    # By at the low y-face equals the cell value (uniform in y); Bc_y = By
+    b[0, 0, :, 1] = -vy
+    p0 = 0.6
+    u[..., 4] = p0 / (5 / 3 - 1) + 0.5 * (u[..., 2] ** 2) + 0.5 * (1.0 + b[..., 1] ** 2)
+    u, b = u.reshape(-1), b.reshape(-1)
+    t, steps = 0.0, 0
+    uu, bb = u.copy(), b.copy()
+    while t < n - 1e-9:
+        uu, bb, t_new, dts = O64.run(uu, bb, n, 1, 1, 1, cfl=0.7, tf=float(n), t=t)
+        t = float(t_new)
+        steps += 1
+        assert steps < 2000
+    err = np.abs(uu[2::5] - u[2::5]).max()
+    assert abs(t - n) < 1e-9
+    assert err < 0.02, err   # 2nd-order scheme, 32 cells per wavelength, one period
