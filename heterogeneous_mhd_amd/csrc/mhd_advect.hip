@@ -34,10 +34,11 @@ __global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out,
     const int pb = BLOCK / tp;
     const int lp = threadIdx.x / tp;
     const int lt = threadIdx.x - lp * tp;
-    const long npen = (long)ny * nz;
-    const long pen = (long)blockIdx.x * pb + lp;
+    const int npen = ny * nz;
+    const int pen = blockIdx.x * pb + lp;
     const bool live = pen < npen;
-    const int j = (int)(pen % ny), k = (int)(pen / ny);
+    const int pc = live ? pen : 0;             // in-range pencil for the clamped loads
+    const int j = pc % ny, k = pc / ny;
     const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
     const int km = k == 0 ? nz - 1 : k - 1, kp = k + 1 == nz ? 0 : k + 1;
     const float dt = dtp ? *dtp : dtv;
@@ -48,38 +49,41 @@ __global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out,
     float *Y = X + kLines * nx;
     float *Z = Y + kLines * nx;
 
-    // pencil (row) offsets in cells
+    // pencil (row) offsets in cells: jm, c, jp, km, kp
     const size_t row = (size_t)nx;
-    const size_t o_c = ((size_t)k * ny + j) * row;
-    const size_t o_jm = ((size_t)k * ny + jm) * row, o_jp = ((size_t)k * ny + jp) * row;
-    const size_t o_km = ((size_t)km * ny + j) * row, o_kp = ((size_t)kp * ny + j) * row;
-    // line t: face velocity from pencils (lo, hi); advected field comp / pencil
-    const size_t lo[kLines] = {o_jm, o_c, o_km, o_c};
-    const size_t hi[kLines] = {o_c, o_jp, o_c, o_kp};
-    const size_t fb[kLines] = {o_c, o_jp, o_c, o_kp};
-    const int comp[kLines] = {1, 1, 2, 2};
+    const size_t op[5] = {((size_t)k * ny + jm) * row, ((size_t)k * ny + j) * row,
+                          ((size_t)k * ny + jp) * row, ((size_t)km * ny + j) * row,
+                          ((size_t)kp * ny + j) * row};
+    // line t: face velocity from pencils (lo, hi); its field comes from pencil hi
+    constexpr int lo[kLines] = {0, 1, 3, 1}, hi[kLines] = {1, 2, 1, 4};
+    constexpr int comp[kLines] = {1, 1, 2, 2};
 
+    // ---- every global load first (indices clamped into range so that the
+    // loads are unconditional and all in flight together), then the raw face
+    // velocity vx = (m1(lo)+m1(hi))/(rho(lo)+rho(hi)), A1 :34 / B1 :34
+    float rho[M][5], m1[M][5];
     float bl[M][kLines];  // the advected face field, per line
     float b1m[M];         // main b1 of this pencil
-    // ---- raw face velocity vx = (m1(lo)+m1(hi))/(rho(lo)+rho(hi)), A1 :34 / B1 :34
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        const int ic = i < nx ? i : nx - 1;
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            rho[r][q] = uu[(op[q] + ic) * 5 + 0];
+            m1[r][q] = uu[(op[q] + ic) * 5 + 1];
+        }
+#pragma unroll
+        for (int t = 0; t < kLines; t++) bl[r][t] = bu[(op[hi[t]] + ic) * 3 + comp[t]];
+        b1m[r] = b_main[(op[1] + ic) * 3 + 0];
+    }
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
         if (live && i < nx) {
-            float cu[2];
-            cu[0] = uu[(o_c + i) * 5 + 0];
-            cu[1] = uu[(o_c + i) * 5 + 1];
 #pragma unroll
-            for (int t = 0; t < kLines; t++) {
-                const size_t a = lo[t] + i, bidx = hi[t] + i;
-                const float rl = (lo[t] == o_c) ? cu[0] : uu[a * 5 + 0];
-                const float ml = (lo[t] == o_c) ? cu[1] : uu[a * 5 + 1];
-                const float rh = (hi[t] == o_c) ? cu[0] : uu[bidx * 5 + 0];
-                const float mh = (hi[t] == o_c) ? cu[1] : uu[bidx * 5 + 1];
-                X[t * nx + i] = (ml + mh) / (rl + rh);
-                bl[r][t] = bu[(fb[t] + i) * 3 + comp[t]];
-            }
-            b1m[r] = b_main[(o_c + i) * 3 + 0];
+            for (int t = 0; t < kLines; t++)
+                X[t * nx + i] = (m1[r][lo[t]] + m1[r][hi[t]]) / (rho[r][lo[t]] + rho[r][hi[t]]);
         }
     }
     __syncthreads();
@@ -189,7 +193,7 @@ __global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out,
             const float b2n = bl[r][0] - (fx[r][0] - fm[0]);
             const float b3n = bl[r][2] - (fx[r][2] - fm[2]);
             const float b1n = (((b1m[r] - fm[0]) + fm[1]) - fm[2]) + fm[3];
-            float *o = b_out + (o_c + i) * 3;
+            float *o = b_out + (op[1] + i) * 3;
             o[0] = b1n;
             o[1] = b2n;
             o[2] = b3n;
@@ -221,12 +225,13 @@ int launch_advectbyzx(float *b_out, const float *b_main, const float *uu, const 
 {
     if (!b_out || !b_main || !uu || !bu || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
     if (nx > kMaxPencil) return MHD_ERR_TOO_LONG;
+    if ((long)ny * nz > 0x7fffffffL) return MHD_ERR_ARG;
     if (nx <= 64) return launch_cfg<256, 1>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
     if (nx <= 128) return launch_cfg<256, 2>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
-    if (nx <= 256) return launch_cfg<256, 4>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
-    if (nx <= 512) return launch_cfg<256, 4>(b_out, b_main, uu, bu, nx, ny, nz, 128, dt_dev, dt_val, s);
-    if (nx <= 1024) return launch_cfg<256, 4>(b_out, b_main, uu, bu, nx, ny, nz, 256, dt_dev, dt_val, s);
-    return launch_cfg<512, 4>(b_out, b_main, uu, bu, nx, ny, nz, 512, dt_dev, dt_val, s);
+    if (nx <= 256) return launch_cfg<256, 2>(b_out, b_main, uu, bu, nx, ny, nz, 128, dt_dev, dt_val, s);
+    if (nx <= 512) return launch_cfg<256, 2>(b_out, b_main, uu, bu, nx, ny, nz, 256, dt_dev, dt_val, s);
+    if (nx <= 1024) return launch_cfg<512, 2>(b_out, b_main, uu, bu, nx, ny, nz, 512, dt_dev, dt_val, s);
+    return launch_cfg<1024, 2>(b_out, b_main, uu, bu, nx, ny, nz, 1024, dt_dev, dt_val, s);
 }
 
 }  // namespace mhd

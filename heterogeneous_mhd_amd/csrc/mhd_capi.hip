@@ -233,7 +233,7 @@ float calcfl(float *u, float *b, int *nx, int *ny, int *nz)
     bool ok = true;
     const float *du = stage_in(u, 5 * n, 0, &ok);
     const float *db = stage_in(b, 3 * n, 1, &ok);
-    if (!ok || !g_cfl.ensure(mhd::cfl_workspace_bytes(n))) return __builtin_nanf("");
+    if (!ok || !g_cfl.ensure(mhd::cfl_workspace_bytes(*nx, *ny, *nz))) return __builtin_nanf("");
     float r = __builtin_nanf("");
     int rc = mhd::launch_calcfl(g_cfl.inv, du, db, *nx, *ny, *nz, g_cfl.ws, 0.0, nullptr, nullptr,
                                 0.0f, 0);
@@ -292,7 +292,7 @@ int mhd_copy_matrix_async(const float *u, const float *b, int nx, int ny, int nz
 size_t mhd_calcfl_workspace_bytes(int nx, int ny, int nz)
 {
     if (nx < 1 || ny < 1 || nz < 1) return 0;
-    return mhd::cfl_workspace_bytes((size_t)nx * ny * nz);
+    return mhd::cfl_workspace_bytes(nx, ny, nz);
 }
 
 int mhd_calcfl_async(float *inv_c_dev, const float *u, const float *b, int nx, int ny, int nz,
@@ -310,11 +310,12 @@ struct mhd_plan {
     double cfl;
     float tf;
     hipStream_t s;
-    float *scal;      // [0]=t, [1]=dt, [2]=1/c
+    float *scal;      // [0]=t, [1]=dt, [2]=1/c, [3]=next step's unclamped dt
     void *ws;         // calcfl workspace
     hipGraph_t graph;
     hipGraphExec_t exec;
     const float *gptr[4];
+    const float *ready_for[2];  // state whose calcfl is in scal[2..3]
 };
 
 float *mhd_plan_t_dev(mhd_plan *p) { return p ? p->scal : nullptr; }
@@ -328,7 +329,7 @@ int mhd_plan_create(mhd_plan **plan, int nx, int ny, int nz, double cfl, float t
     if (nx > mhd::kMaxPencil || ny > mhd::kMaxPencil) return MHD_ERR_TOO_LONG;
     mhd_plan *p = new mhd_plan();
     p->nx = nx, p->ny = ny, p->nz = nz, p->cfl = cfl, p->tf = tf, p->s = (hipStream_t)stream;
-    const size_t wsb = mhd::cfl_workspace_bytes((size_t)nx * ny * nz);
+    const size_t wsb = mhd::cfl_workspace_bytes(nx, ny, nz);
     if (hipMalloc(&p->scal, 64) != hipSuccess || hipMalloc(&p->ws, wsb) != hipSuccess) {
         note(hipGetLastError(), "plan alloc");
         delete p;
@@ -362,6 +363,7 @@ int mhd_plan_set_time(mhd_plan *p, float t)
     hipError_t e = hipMemcpyAsync(p->scal, &t, sizeof(float), hipMemcpyHostToDevice, p->s);
     if (e == hipSuccess) e = hipStreamSynchronize(p->s);
     note(e, "set_time");
+    p->ready_for[0] = p->ready_for[1] = nullptr;  // the state may have changed too
     return e == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -369,18 +371,20 @@ int mhd_plan_set_time(mhd_plan *p, float t)
 // copy_matrix calls disappear into a ping-pong between (U0,B0) and (U1,B1).
 // fluidx works in place; advectbyzx and the transposes write the other
 // buffer.  u changes buffer 4 times and b 10 times per step, so the state
-// ends where it started.
+// ends where it started.  calcfl (:62) of the next step is fused into this
+// step's last fluid sweep, which writes 1/c and the unclamped dt to scal[2..3];
+// the step begins by applying the clamp and advancing t (:63-64).
 static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
 {
     const int nx = p->nx, ny = p->ny, nz = p->nz;
     const hipStream_t s = p->s;
-    float *t = p->scal, *dt = p->scal + 1, *inv = p->scal + 2;
+    float *t = p->scal, *dt = p->scal + 1, *inv = p->scal + 2, *dtn = p->scal + 3;
     int rc = MHD_OK;
 #define RUN(x)                    \
     do {                          \
         if (rc == MHD_OK) rc = (x); \
     } while (0)
-    RUN(mhd::launch_calcfl(inv, U0, B0, nx, ny, nz, p->ws, p->cfl, dt, t, p->tf, s));  // :62-64
+    RUN(mhd::launch_begin_step(t, dt, dtn, p->tf, s));                                  // :63-64
     // x sweep
     RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s));                       // :71
     RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, nx, ny, nz, dt, 0.0f, s));               // :74
@@ -399,9 +403,9 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
     RUN(mhd::launch_advectbyzx(B0, B1, U1, B1, ny, nx, nz, dt, 0.0f, s));               // :105
     RUN(mhd::launch_fluidx(U1, U1, B0, ny, nx, nz, dt, 0.0f, s));                       // :107
     RUN(mhd::launch_transpose12(U0, B1, U1, B0, ny, nx, nz, s));                         // :112
-    // x again
+    // x again; its sweep also reduces calcfl of the new state for the next step
     RUN(mhd::launch_advectbyzx(B0, B1, U0, B1, nx, ny, nz, dt, 0.0f, s));               // :115
-    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s));                       // :117
+    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl)); // :117
 #undef RUN
     return rc;
 }
@@ -409,6 +413,14 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
 int mhd_plan_run(mhd_plan *p, float *u, float *b, float *u2, float *b2, int nsteps, int use_graph)
 {
     if (!p || !u || !b || !u2 || !b2 || nsteps < 0) return MHD_ERR_ARG;
+    if (p->ready_for[0] != u || p->ready_for[1] != b) {
+        // first step on this state: calcfl (:62) on its own
+        int rc = mhd::launch_calcfl(p->scal + 2, u, b, p->nx, p->ny, p->nz, p->ws, p->cfl,
+                                    p->scal + 3, nullptr, 0.0f, p->s);
+        if (rc != MHD_OK) return rc;
+        p->ready_for[0] = u;
+        p->ready_for[1] = b;
+    }
     if (!use_graph) {
         for (int i = 0; i < nsteps; i++) {
             int rc = enqueue_step(p, u, b, u2, b2);

@@ -4,23 +4,39 @@
 // mhdflux/find_max C/mhdflux.c:9-72.
 //
 // Mapping.  One pencil (all nx cells of one (j,k)) is owned by `tp` threads
-// (a multiple of the 64-wide wavefront); a 256- or 512-thread workgroup owns
-// blockDim/tp consecutive pencils, i.e. one contiguous run of u in HBM, which
-// it loads and stores fully coalesced through LDS.  Thread t of a pencil holds
-// cells t, t+tp, t+2tp, ... in registers.  Stencil neighbours (radius 3 over
-// the two TVD stages) go through two LDS arrays of 5 floats per cell, stored
-// cell-major: a stride of 5 words is odd, so the per-cell reads of a
-// wavefront are free of bank conflicts.  The pencil-global freezing speed c
+// (a multiple of the 64-wide wavefront); a workgroup owns BLOCK/tp
+// consecutive pencils, i.e. one contiguous run of u in HBM, which it stages
+// into LDS with 16-byte loads (all issued before the first LDS store) and
+// writes back the same way.  Thread t of a pencil holds cells t, t+tp, ... in
+// registers.  Stencil neighbours go through two LDS arrays of 5 floats per
+// cell, cell-major: a stride of 5 words is odd, so a wavefront's per-cell
+// reads are free of bank conflicts.  The pencil-global freezing speed c
 // (find_max, twice per sweep) is a wavefront butterfly plus, when a pencil
 // spans several wavefronts, one LDS exchange.
 //
-// Arithmetic.  Every expression repeats the reference's float/double
-// pattern (float storage, double wherever a double literal or libm double
-// routine appears) and the library is built with -ffp-contract=off, so the
-// result is bitwise that of the reference compiled by gcc on x86-64.
-// Each "+ z" is the reference's abXYc "+ c" with c = 0 (C/includefunc_spu.h:80);
-// it maps -0 to +0 and is kept on purpose.
-#include "mhd_internal.h"
+// Arithmetic.  Every result is bitwise the reference's (float storage,
+// double wherever a double literal or a libm double routine appears, no FMA
+// contraction: the library is built with -ffp-contract=off).  The sweep is
+// VALU-bound, so the reference's operations are regrouped where the rounded
+// results are provably identical:
+//  * abXYc(L, R, R, c, 0, 0) = c*R + 0*R + 0 (C/includefunc_spu.h:80) is
+//    computed as c*R + (R - R): both give c*R with -0 mapped to +0 for finite
+//    R and NaN for infinite or NaN R.
+//  * dfrp(i) and dfrm(i+1) (C/tvd1.c:60-64) are the same rounded expression
+//    of fr(i+1), fr(i); likewise dflm(i+1) and dflp(i) (:86-90); and
+//    fl(i) = c*wl(i+1) (:83-84).  Each is computed once per cell and shared
+//    through LDS instead of being recomputed by three neighbours.
+//  * x / c for the pencil-constant c (C/mhdflux.c:54) is x*r corrected by one
+//    FMA residual step with r = RN(1/c); with |x| in [2^-60, 2^60] and c in
+//    [2^-30, 2^30] that is the correctly rounded quotient (Markstein's
+//    theorem for an exactly rounded reciprocal); zeros, tiny, huge and
+//    non-finite x and any c outside that range use the IEEE division.
+// The limiter 2ab/(a+b) (:78, :104) and the double square root stay IEEE.
+//
+// With a CFL workspace the kernel also computes calcfl (C/calcfl.c:13-63) of
+// the state it writes, so the last sweep of a step yields the next step's dt
+// without another pass over HBM.
+#include "mhd_cfl_common.h"
 
 namespace mhd {
 namespace {
@@ -54,6 +70,20 @@ __device__ __forceinline__ float limiter(float a, float b)
     return r;
 }
 
+// c*R + 0*R + 0 of the reference, see the file header.
+__device__ __forceinline__ float scale0(float c, float x) { return c * x + (x - x); }
+
+// x / c, correctly rounded; r = 1/c, fast = c in [2^-30, 2^30] (uniform).
+__device__ __forceinline__ float div_c(float x, float c, float r, bool fast)
+{
+    if (!fast) return x / c;
+    const float q0 = x * r;
+    float q = fmaf(fmaf(-q0, c, x), r, q0);
+    const float ax = fabsf(x);
+    if (__builtin_expect(!(ax >= 0x1p-60f && ax <= 0x1p60f), 0)) q = (x == 0.0f) ? q0 : x / c;
+    return q;
+}
+
 // Max over the threads of one pencil.  x >= 0 and never NaN (the callers
 // start from 0 and keep the running max with a strict '>', as find_max does,
 // so a NaN candidate never enters).  Ends with a workgroup barrier, which the
@@ -81,24 +111,63 @@ __device__ __forceinline__ float pencil_max(float x, int tp, float *red)
     return x;
 }
 
-template <int BLOCK, int M>
-__global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
-                                                       const float *uu,
+// Copies na floats a_src -> a_dst and nb floats b_src -> b_dst (global ->
+// LDS).  With vec, 16-byte vectors: all loads of both arrays are issued
+// before the first LDS store.  Loads and stores use an index clamped into
+// range (a clamped lane rewrites the last vector with its own, identical
+// value), so there is no branch for the compiler to sink the loads into.
+template <int BLOCK, int KA, int KB>
+__device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int na, float *b_dst,
+                                         const float *b_src, int nb, bool vec)
+{
+    if (vec) {
+        const float4 *sa = reinterpret_cast<const float4 *>(a_src);
+        const float4 *sb = reinterpret_cast<const float4 *>(b_src);
+        const int na4 = na >> 2, nb4 = nb >> 2;
+        float4 ra[KA], rb[KB];
+#pragma unroll
+        for (int k = 0; k < KA; k++) {
+            const int f = threadIdx.x + k * BLOCK;
+            ra[k] = sa[f < na4 ? f : na4 - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int f = threadIdx.x + k * BLOCK;
+            rb[k] = sb[f < nb4 ? f : nb4 - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < KA; k++) {
+            const int f = threadIdx.x + k * BLOCK;
+            reinterpret_cast<float4 *>(a_dst)[f < na4 ? f : na4 - 1] = ra[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int f = threadIdx.x + k * BLOCK;
+            reinterpret_cast<float4 *>(b_dst)[f < nb4 ? f : nb4 - 1] = rb[k];
+        }
+    } else {
+        for (int f = threadIdx.x; f < na; f += BLOCK) a_dst[f] = a_src[f];
+        for (int f = threadIdx.x; f < nb; f += BLOCK) b_dst[f] = b_src[f];
+    }
+}
+
+template <int BLOCK, int M, bool CFL>
+__global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float *uu,
                                                        const float *__restrict__ bu, int nx, int ny,
                                                        int nz, int tp, const float *__restrict__ dtp,
-                                                       float dtv)
+                                                       float dtv, CflOut co)
 {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float red[2][BLOCK / 64];
     const int pb = BLOCK / tp;                 // pencils per workgroup
     const int lp = threadIdx.x / tp;           // this thread's pencil in the group
     const int lt = threadIdx.x - lp * tp;      // this thread's rank in its pencil
-    const long npen = (long)ny * nz;
-    const long pen0 = (long)blockIdx.x * pb;
-    const int npb = (int)((npen - pen0) < pb ? (npen - pen0) : pb);  // pencils present
-    const long pen = pen0 + lp;
+    const int npen = ny * nz;
+    const int pen0 = blockIdx.x * pb;
+    const int npb = npen - pen0 < pb ? npen - pen0 : pb;  // pencils present
+    const int pen = pen0 + lp;
     const bool live = lp < npb;
-    const int j = (int)(pen % ny), k = (int)(pen / ny);
+    const int j = pen % ny, k = pen / ny;
     const int jp = j + 1 == ny ? 0 : j + 1, kp = k + 1 == nz ? 0 : k + 1;  // C/fluidx.c:52-53
 
     float *A = lds;                            // 5*nx*pb floats
@@ -107,18 +176,28 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
 
     const float dt = dtp ? *dtp : dtv;
     const float z = 0.0f, h = 0.5f;
+    constexpr int KU = (5 * M + 3) / 4, KB = (3 * M + 3) / 4;  // 16-byte loads per thread
 
-    // ---- coalesced load of the group's u (5 floats/cell) and b (3/cell)
-    {
-        const float *gu = uu + (size_t)pen0 * nx * 5;
-        const float *gb = bu + (size_t)pen0 * nx * 3;
-        const int nu = 5 * nx * npb, nb = 3 * nx * npb;
-        for (int f = threadIdx.x; f < nu; f += BLOCK) A[f] = gu[f];
-        for (int f = threadIdx.x; f < nb; f += BLOCK) B[f] = gb[f];
+    // ---- stage the group's u (5 floats/cell) and b (3/cell) into LDS; the
+    // neighbour faces b2(j+1), b3(k+1) go straight to registers
+    const float *gu = uu + (size_t)pen0 * nx * 5;
+    const float *gb = bu + (size_t)pen0 * nx * 3;
+    const int nu = 5 * nx * npb, nb = 3 * nx * npb;
+    const bool vec = ((uintptr_t)gu % 16 == 0) && ((uintptr_t)gb % 16 == 0) && nu % 4 == 0 &&
+                     nb % 4 == 0 && (5 * nx * pb) % 4 == 0;
+    float b2jp[M], b3kp[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            b2jp[r] = bu[(((size_t)k * ny + jp) * nx + i) * 3 + 1];
+            b3kp[r] = bu[(((size_t)kp * ny + j) * nx + i) * 3 + 2];
+        }
     }
+    stage_in<BLOCK, KU, KB>(A, gu, nu, B, gb, nb, vec);
     __syncthreads();
 
-    float q[M][5], bc[M][3], v[M][5];
+    float q[M][5], bc[M][3], v[M][5], bf[CFL ? M : 1][3];
     float cmax = 0.0f;
     const float *Bs = B + 3 * nx * lp;         // b staging of this pencil (3 floats/cell)
 #pragma unroll
@@ -128,13 +207,13 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
             const int ip = i + 1 == nx ? 0 : i + 1;
 #pragma unroll
             for (int c = 0; c < 5; c++) q[r][c] = Ap[5 * i + c];
+            if (CFL)
+                for (int c = 0; c < 3; c++) bf[r][c] = Bs[3 * i + c];
             // cell-centred B, C/fluidx.c:71-81
             const float h1 = h * Bs[3 * i], h1p = h * Bs[3 * ip];
-            const float b2jp = bu[(((size_t)k * ny + jp) * nx + i) * 3 + 1];
-            const float b3kp = bu[(((size_t)kp * ny + j) * nx + i) * 3 + 2];
             bc[r][0] = h1 + h1p;
-            bc[r][1] = h * Bs[3 * i + 1] + b2jp / 2;
-            bc[r][2] = h * Bs[3 * i + 2] + b3kp / 2;
+            bc[r][1] = h * Bs[3 * i + 1] + b2jp[r] / 2;
+            bc[r][2] = h * Bs[3 * i + 2] + b3kp[r] / 2;
             const float cf = flux_cell(q[r], bc[r], v[r]);
             cmax = cf > cmax ? cf : cmax;
         }
@@ -142,20 +221,25 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
     const float c1 = pencil_max<BLOCK>(cmax, tp, red[0]);
 
     // ---- stage 1: half-step predictor, C/tvd1.c:36-49
-    // s[][] holds, in turn, fr of stage 1, the predictor u1, and the final
-    // flux; q keeps the sweep's input u, the base of the final update (:116).
+    // s[][] holds, in turn, fr of stage 1, the predictor u1, fr of stage 2 and
+    // the final flux; q keeps the sweep's input u, the base of the update (:116).
     float s[M][5];
+    {
+        const bool norm = c1 > 0.0f;
+        const bool fast = c1 >= 0x1p-30f && c1 <= 0x1p30f;
+        const float rc = norm ? 1.0f / c1 : 0.0f;
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
 #pragma unroll
-            for (int c = 0; c < 5; c++) {
-                float vv = v[r][c];
-                if (c1 > 0.0f) vv = vv / c1;
-                const float wr = q[r][c] + vv + z;
-                Ap[5 * i + c] = q[r][c] - vv + z;           // wl
-                s[r][c] = c1 * wr + z * wr + z;   // fr
+                for (int c = 0; c < 5; c++) {
+                    const float vv = norm ? div_c(v[r][c], c1, rc, fast) : v[r][c];
+                    const float wr = q[r][c] + vv + z;
+                    const float wl = q[r][c] - vv + z;
+                    s[r][c] = scale0(c1, wr);          // fr
+                    Ap[5 * i + c] = scale0(c1, wl);    // fl(i-1) = c*wl(i)
+                }
             }
         }
     }
@@ -166,11 +250,7 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
         if (live && i < nx) {
             const int ip = i + 1 == nx ? 0 : i + 1;
 #pragma unroll
-            for (int c = 0; c < 5; c++) {
-                const float t = Ap[5 * ip + c];
-                const float fl = c1 * t + z * t + z;
-                Bp[5 * i + c] = h * s[r][c] + (-h) * fl + z;  // flux
-            }
+            for (int c = 0; c < 5; c++) Bp[5 * i + c] = h * s[r][c] + (-h) * Ap[5 * ip + c] + z;  // flux
         }
     }
     __syncthreads();
@@ -193,17 +273,53 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
     const float c2 = pencil_max<BLOCK>(cmax, tp, red[1]);
 
     // ---- stage 2: corrector with the van Leer limiter, C/tvd1.c:54-116
+    {
+        const bool norm = c2 > 0.0f;
+        const bool fast = c2 >= 0x1p-30f && c2 <= 0x1p30f;
+        const float rc = norm ? 1.0f / c2 : 0.0f;
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) {
+                    const float vv = norm ? div_c(v[r][c], c2, rc, fast) : v[r][c];
+                    const float wr = s[r][c] + vv + z;
+                    const float wl = s[r][c] - vv + z;
+                    s[r][c] = scale0(c2, wr);          // fr
+                    Ap[5 * i + c] = s[r][c];
+                    Bp[5 * i + c] = scale0(c2, wl);    // fl(i-1)
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // D(i) = dfrm(i) = dfrp(i-1); E(i) = dflm(i) = dflp(i-1); t1 = fr - fl
+    float D[M][5], E[M][5];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                const float fr = s[r][c], frm = Ap[5 * im + c];
+                const float flm = Bp[5 * i + c], fl0 = Bp[5 * ip + c];
+                D[r][c] = h * fr + (-h) * frm + z;
+                E[r][c] = h * flm + (-h) * fl0 + z;
+                s[r][c] = fr - fl0 + z;                // t1
+            }
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
         if (live && i < nx) {
 #pragma unroll
             for (int c = 0; c < 5; c++) {
-                float vv = v[r][c];
-                if (c2 > 0.0f) vv = vv / c2;
-                const float wr = s[r][c] + vv + z;
-                Ap[5 * i + c] = c2 * wr + z * wr + z;      // fr
-                Bp[5 * i + c] = s[r][c] - vv + z;          // wl
+                Ap[5 * i + c] = D[r][c];
+                Bp[5 * i + c] = E[r][c];
             }
         }
     }
@@ -213,24 +329,12 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
         const int i = lt + r * tp;
         if (live && i < nx) {
             const int ip = i + 1 == nx ? 0 : i + 1;
-            const int ipp = ip + 1 == nx ? 0 : ip + 1;
-            const int im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
             for (int c = 0; c < 5; c++) {
-                const float frm = Ap[5 * im + c], fr0 = Ap[5 * i + c], frp = Ap[5 * ip + c];
-                const float dfrp = h * frp + (-h) * fr0 + z;
-                const float dfrm = h * fr0 + (-h) * frm + z;
-                const float dfr = limiter(dfrp, dfrm);
-                const float w0 = Bp[5 * i + c], w1 = Bp[5 * ip + c], w2 = Bp[5 * ipp + c];
-                const float flm = c2 * w0 + z * w0 + z;     // fl(i-1)
-                const float fl0 = c2 * w1 + z * w1 + z;     // fl(i)
-                const float flp = c2 * w2 + z * w2 + z;     // fl(i+1)
-                const float dflp = h * fl0 + (-h) * flp + z;
-                const float dflm = h * flm + (-h) * fl0 + z;
-                const float dfl = limiter(dflp, dflm);
-                const float t1 = fr0 - fl0 + z;
+                const float dfr = limiter(Ap[5 * ip + c], D[r][c]);   // (dfrp, dfrm)
+                const float dfl = limiter(Bp[5 * ip + c], E[r][c]);   // (dflp, dflm)
                 const float t2 = dfr - dfl + z;
-                s[r][c] = h * t1 + h * t2 + z;              // flux
+                s[r][c] = h * s[r][c] + h * t2 + z;                   // flux
             }
         }
     }
@@ -245,56 +349,132 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out,
     }
     __syncthreads();
     const float fdt = (float)(-1.0 * (double)dt);
+    float tcf[M];
+    int toff[M];
+    int lnan = -1;
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
+        tcf[r] = -INFINITY;
+        toff[r] = lp * nx + i;
         if (live && i < nx) {
             const int im = i == 0 ? nx - 1 : i - 1;
+            float un[5];
 #pragma unroll
             for (int c = 0; c < 5; c++) {
                 const float d = Ap[5 * i + c] - Ap[5 * im + c] + z;
-                Bp[5 * i + c] = q[r][c] + fdt * d + z;
+                un[c] = q[r][c] + fdt * d + z;
+                Bp[5 * i + c] = un[c];
+            }
+            if (CFL) {
+                tcf[r] = cfl_cell(un, bf[r]);
+                if (tcf[r] != tcf[r]) lnan = toff[r] > lnan ? toff[r] : lnan;
             }
         }
     }
     __syncthreads();
     {
-        float *gu = u_out + (size_t)pen0 * nx * 5;
-        const int nu = 5 * nx * npb;
-        for (int f = threadIdx.x; f < nu; f += BLOCK) gu[f] = B[f];
+        float *go = u_out + (size_t)pen0 * nx * 5;
+        if (vec) {
+            const float4 *s4 = reinterpret_cast<const float4 *>(B);
+            float4 *d4 = reinterpret_cast<float4 *>(go);
+            const int n4 = nu >> 2;
+#pragma unroll
+            for (int kk = 0; kk < KU; kk++) {
+                const int f = threadIdx.x + kk * BLOCK;
+                if (f < n4) d4[f] = s4[f];
+            }
+        } else {
+            for (int f = threadIdx.x; f < nu; f += BLOCK) go[f] = B[f];
+        }
     }
+    if (CFL) cfl_block_record<BLOCK, M>(tcf, toff, lnan, co.ws);
 }
 
-template <int BLOCK, int M>
+struct Cfg {
+    int block, m, tp;
+};
+
+Cfg pick(int nx)
+{
+    if (nx <= 64) return {256, 1, 64};
+    if (nx <= 128) return {256, 2, 64};
+    if (nx <= 256) return {256, 2, 128};
+    if (nx <= 512) return {256, 2, 256};
+    if (nx <= 1024) return {512, 2, 512};
+    return {1024, 2, 1024};
+}
+
+template <int BLOCK, int M, bool CFL>
 int launch_cfg(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz, int tp,
-               const float *dtp, float dtv, hipStream_t s)
+               const float *dtp, float dtv, const CflOut &co, hipStream_t s)
 {
     const int pb = BLOCK / tp;
     const long npen = (long)ny * nz;
     const long nblk = (npen + pb - 1) / pb;
     const size_t lds = sizeof(float) * 10 * (size_t)nx * pb;
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)fluidx_kernel<BLOCK, M>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void *)fluidx_kernel<BLOCK, M, CFL>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
-    hipLaunchKernelGGL((fluidx_kernel<BLOCK, M>), dim3((unsigned)nblk), dim3(BLOCK), lds, s, u_out,
-                       uu, bu, nx, ny, nz, tp, dtp, dtv);
+    hipLaunchKernelGGL((fluidx_kernel<BLOCK, M, CFL>), dim3((unsigned)nblk), dim3(BLOCK), lds, s,
+                       u_out, uu, bu, nx, ny, nz, tp, dtp, dtv, co);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+template <bool CFL>
+int launch_any(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
+               const float *dtp, float dtv, const CflOut &co, hipStream_t s)
+{
+    const Cfg c = pick(nx);
+    if (c.block == 256 && c.m == 1) return launch_cfg<256, 1, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    if (c.block == 256) return launch_cfg<256, 2, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    if (c.block == 512) return launch_cfg<512, 2, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    return launch_cfg<1024, 2, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
 }
 
 }  // namespace
 
+size_t fluid_workgroups(int nx, long npen)
+{
+    const Cfg c = pick(nx);
+    const int pb = c.block / c.tp;
+    return (size_t)((npen + pb - 1) / pb);
+}
+
 int launch_fluidx(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
-                  const float *dt_dev, float dt_val, hipStream_t s)
+                  const float *dt_dev, float dt_val, hipStream_t s, void *cfl_ws, float *inv_c,
+                  float *dt_next, double cfl)
 {
     if (!u_out || !uu || !bu || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
     if (nx > kMaxPencil) return MHD_ERR_TOO_LONG;
-    if (nx <= 64) return launch_cfg<256, 1>(u_out, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
-    if (nx <= 128) return launch_cfg<256, 2>(u_out, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
-    if (nx <= 256) return launch_cfg<256, 4>(u_out, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
-    if (nx <= 512) return launch_cfg<256, 4>(u_out, uu, bu, nx, ny, nz, 128, dt_dev, dt_val, s);
-    if (nx <= 1024) return launch_cfg<256, 4>(u_out, uu, bu, nx, ny, nz, 256, dt_dev, dt_val, s);
-    return launch_cfg<512, 4>(u_out, uu, bu, nx, ny, nz, 512, dt_dev, dt_val, s);
+    if ((long)ny * nz > 0x7fffffffL) return MHD_ERR_ARG;
+    CflOut co = {cfl_ws, inv_c, dt_next, nullptr, cfl, 0.0f};
+    if (cfl_ws) {
+        if (!inv_c) return MHD_ERR_ARG;
+        const int rc = launch_any<true>(u_out, uu, bu, nx, ny, nz, dt_dev, dt_val, co, s);
+        if (rc != MHD_OK) return rc;
+        return launch_cfl_fold((int)fluid_workgroups(nx, (long)ny * nz), co, s);
+    }
+    return launch_any<false>(u_out, uu, bu, nx, ny, nz, dt_dev, dt_val, co, s);
+}
+
+namespace {
+__global__ void begin_step_kernel(float *t, float *dt, const float *dt_next, float tf)
+{
+    float d = *dt_next;
+    const float t0 = *t;
+    if (tf > 0.0f && d > (tf - t0) / 2) d = (tf - t0) / 2;   // C/mhd.c:63
+    *t = t0 + 2 * d;                                         // C/mhd.c:64
+    *dt = d;
+}
+}  // namespace
+
+int launch_begin_step(float *t, float *dt, const float *dt_next, float tf, hipStream_t s)
+{
+    (void)hipGetLastError();  // clear a stale error so the check below sees this launch
+    hipLaunchKernelGGL(begin_step_kernel, dim3(1), dim3(1), 0, s, t, dt, dt_next, tf);
+    return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
 }  // namespace mhd

@@ -25,8 +25,15 @@ namespace mhd {
 // Largest pencil (first-axis length) the sweep kernels accept.
 constexpr int kMaxPencil = 2048;
 
+// When cfl_ws is non-null the kernel also computes calcfl of its output
+// state (u_out with the face field bu) and writes inv_c / dt_next exactly as
+// launch_calcfl does (no clamp, no t update).
 int launch_fluidx(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
-                  const float *dt_dev, float dt_val, hipStream_t s);
+                  const float *dt_dev, float dt_val, hipStream_t s, void *cfl_ws = nullptr,
+                  float *inv_c = nullptr, float *dt_next = nullptr, double cfl = 0.0);
+
+// dt = dt_next, clamped to (tf - t)/2 when tf > 0; t += 2 dt  (C/mhd.c:63-64)
+int launch_begin_step(float *t, float *dt, const float *dt_next, float tf, hipStream_t s);
 
 int launch_advectbyzx(float *b_out, const float *b_main, const float *uu, const float *bu, int nx,
                       int ny, int nz, const float *dt_dev, float dt_val, hipStream_t s);
@@ -37,9 +44,13 @@ int launch_transpose12(float *u_out, float *b_out, const float *uu, const float 
 int launch_transpose13(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                        int nz, hipStream_t s);
 
-// CFL reduction workspace: partial records + a completion counter.  The
-// counter must be zero before the first launch; the kernel leaves it zero.
-size_t cfl_workspace_bytes(size_t ncell);
+// CFL reduction workspace: a ticket counter + one record per workgroup.  The
+// counter must be zero before the first launch; the kernels leave it zero.
+// Sized for both the stand-alone kernel and the x-sweep fluid kernel that
+// fuses the reduction (launch_fluidx with cfl != null).
+size_t cfl_workspace_bytes(int nx, int ny, int nz);
+// Workgroups of launch_fluidx for pencils of length nx.
+size_t fluid_workgroups(int nx, long npen);
 // Writes inv_c[0] = 1/c (float, the reference's calcfl return value).  When
 // dt_dev is non-null also applies the driver's dt rule on the device:
 //   dt = (float)(cfl * (double)inv_c); if (tf > 0 && dt > (tf - t)/2) dt = (tf - t)/2;
