@@ -10,7 +10,7 @@ stay in HBM.  Prints one JSON line:
   roofline   the dominant kernel's algorithmic bytes per launch / its mean
              duration (HIP events on the launch stream) vs 8 TB/s
   cpu_baseline  the oracle restatement (bitwise equal to the reference C
-             path) timed on this host's cores on a bounded 128^3 sample
+             path) timed on this host's cores on a bounded 256^3 sample
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024]
 """
@@ -62,27 +62,44 @@ def kernel_times(box, reps: int):
     return out
 
 
-def cpu_baseline(budget_s: float = 15.0):
-    """Oracle (FP32, bitwise equal to the reference C path) on host cores."""
+def cpu_baseline(u, b, n: int, budget_s: float = 12.0):
+    """Oracle (FP32, bitwise equal to the reference C path) on host cores,
+    stepping the given n^3 state (host arrays) for about budget_s seconds."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from heterogeneous_mhd_amd import solver
     from oracle import Oracle
     threads = min(16, os.cpu_count() or 1)
-    n = 128
-    u, b = solver.ic_host(n, n, n, solver.ICSpec(seed=1, amp_v=0.3, amp_a=0.05))
     o = Oracle("f32", threads=threads)
-    o.run(u, b, n, n, n, 1)  # warm-up step
+    u, b, _, _ = o.run(u, b, n, n, n, 1)  # warm-up step
     steps, t0 = 0, time.perf_counter()
     while True:
         u, b, _, _ = o.run(u, b, n, n, n, 1)
         steps += 1
         el = time.perf_counter() - t0
-        if el > budget_s or steps >= 6:
+        if el > budget_s or steps >= 50:
             break
     return {"value": n ** 3 * steps / el, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle mhd_ref.c FP32 (bitwise equal to the reference C path), OpenMP over "
-                      f"pencils, {n}^3 box, {steps} full steps after 1 warm-up, {el:.1f} s"}
+                      f"pencils on {threads} threads, {n}^3 box of the same IC generator, {steps} "
+                      f"full steps after 1 warm-up, {el:.1f} s"}
+
+
+def copy_probe(box, reps: int = 3):
+    """Achieved bandwidth of a plain device copy of the state (copy_matrix):
+    64 bytes per cell moved.  Reported next to the vendor peak."""
+    import torch
+    from heterogeneous_mhd_amd import ops
+    n = box.nx
+    with torch.cuda.stream(box.stream):
+        ops.copy_matrix(box.u, box.b, n, n, n, box.u2, box.b2)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ops.copy_matrix(box.u, box.b, n, n, n, box.u2, box.b2)
+        e1.record()
+        torch.cuda.synchronize()
+    s = e0.elapsed_time(e1) / 1e3 / reps
+    return 64.0 * box.n / s / 1e9
 
 
 def main():
@@ -132,6 +149,7 @@ def main():
     achieved = ALGO_BYTES[dom] * N / kt[dom] / 1e9
     step_bytes = sum(ALGO_BYTES[k] * CALLS_PER_STEP[k] for k in ALGO_BYTES) * N
     diag = box.diagnostics()
+    copy_gbs = copy_probe(box)
     line = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -147,13 +165,20 @@ def main():
                           "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
                           "frac": step_bytes / (wall / args.steps) / 1e9 / PEAK_HBM_GBS},
         "kernel_ms": {k: 1e3 * v for k, v in kt.items()},
+        "copy_GBs": copy_gbs,
         "divb_max": diag["divb_max"],
     }
     del box
     torch.cuda.empty_cache()
     if not args.no_cpu:
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline()
+        # the CPU leg steps a 256^3 box of the same generator (the GPU makes
+        # the IC, bitwise equal to the host generator's)
+        nc = min(n, 256)
+        print(f"[bench] CPU baseline at {nc}^3", file=sys.stderr, flush=True)
+        uc, bc = solver.ic_device(nc, nc, nc, solver.ICSpec(seed=1, amp_v=args.amp_v,
+                                                            amp_a=args.amp_a))
+        torch.cuda.synchronize()
+        line["cpu_baseline"] = cpu_baseline(uc.cpu().numpy(), bc.cpu().numpy(), nc)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out,
     const int lp = threadIdx.x / tp;
     const int lt = threadIdx.x - lp * tp;
     const int npen = ny * nz;
-    const int pen = blockIdx.x * pb + lp;
+    const int pen = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz) * pb + lp;
     const bool live = pen < npen;
     const int pc = live ? pen : 0;             // in-range pencil for the clamped loads
     const int j = pc % ny, k = pc / ny;

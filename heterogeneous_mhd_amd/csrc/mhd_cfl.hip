@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kCB) void cfl_kernel(const float *__restrict__ u,
             if (t[r] != t[r]) lnan = off[r];
         }
     }
-    cfl_block_record<kCB, kCPer>(t, off, lnan, ws);
+    cfl_block_record<kCB, kCPer>(t, off, lnan, ws, blockIdx.x);
 }
 
 // Ordered tree over the workgroup: thread t's summary precedes thread t+s's.

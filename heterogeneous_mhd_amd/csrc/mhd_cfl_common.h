@@ -102,10 +102,11 @@ __device__ __forceinline__ float cfl_block_max_f(float x, float *sh)
 // Called by every thread of a producer workgroup: tval[] are this thread's
 // cell values with their offsets toff[] in the workgroup's run (cells that
 // do not exist hold -inf), lnan the offset of its last NaN cell (-1 if none).
-// Writes the workgroup's record.  Uses ~48 bytes of static LDS.
+// Writes the workgroup's record at index rec, the position of its run in
+// memory order.  Uses ~48 bytes of static LDS.
 template <int BLOCK, int K>
 __device__ __forceinline__ void cfl_block_record(const float (&tval)[K], const int (&toff)[K],
-                                                 int lnan, void *ws)
+                                                 int lnan, void *ws, int rec)
 {
     __shared__ int shi[BLOCK / 64];
     __shared__ float shf[BLOCK / 64];
@@ -116,10 +117,10 @@ __device__ __forceinline__ void cfl_block_record(const float (&tval)[K], const i
         if (toff[r] > L && tval[r] > m) m = tval[r];
     m = cfl_block_max_f<BLOCK>(m, shf);
     if (threadIdx.x == 0) {
-        CflRec rec;
-        rec.has_nan = L >= 0;
-        rec.after = m;
-        cfl_level1(ws)[blockIdx.x] = rec;
+        CflRec r;
+        r.has_nan = L >= 0;
+        r.after = m;
+        cfl_level1(ws)[rec] = r;
     }
 }
 

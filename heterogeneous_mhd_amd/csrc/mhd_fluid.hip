@@ -163,7 +163,8 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float
     const int lp = threadIdx.x / tp;           // this thread's pencil in the group
     const int lt = threadIdx.x - lp * tp;      // this thread's rank in its pencil
     const int npen = ny * nz;
-    const int pen0 = blockIdx.x * pb;
+    const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
+    const int pen0 = grp * pb;
     const int npb = npen - pen0 < pb ? npen - pen0 : pb;  // pencils present
     const int pen = pen0 + lp;
     const bool live = lp < npb;
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float
             for (int f = threadIdx.x; f < nu; f += BLOCK) go[f] = B[f];
         }
     }
-    if (CFL) cfl_block_record<BLOCK, M>(tcf, toff, lnan, co.ws);
+    if (CFL) cfl_block_record<BLOCK, M>(tcf, toff, lnan, co.ws, grp);
 }
 
 struct Cfg {

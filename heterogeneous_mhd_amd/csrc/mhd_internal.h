@@ -25,6 +25,29 @@ namespace mhd {
 // Largest pencil (first-axis length) the sweep kernels accept.
 constexpr int kMaxPencil = 2048;
 
+// Workgroup -> pencil-group schedule of the sweep kernels.  A group is `pb`
+// consecutive pencils along j (one contiguous run of cells); there are
+// G = gj * nz groups with gj = ny / pb.  Workgroups are dispatched round-robin
+// over the 8 XCDs of an MI355X, each with its own L2, so (a) every XCD gets a
+// contiguous share of the schedule, and (b) the schedule walks 8 x 8 tiles of
+// groups in (j, k): the neighbour pencils j+-1 and k+-1 a sweep reads are then
+// read by the same XCD at about the same time and hit its L2.  Shapes that do
+// not divide evenly keep the plain order.  Returns the group index in memory
+// order.
+__device__ __forceinline__ int sweep_group(int bid, int G, int gj, int nz)
+{
+    constexpr int kXcd = 8, kT = 8;
+    int s = bid;
+    if (G % kXcd == 0) s = (bid % kXcd) * (G / kXcd) + bid / kXcd;
+    if (gj % kT == 0 && nz % kT == 0 && G == gj * nz) {
+        const int tile = s / (kT * kT), w = s % (kT * kT);
+        const int tiles_j = gj / kT;
+        const int tj = tile % tiles_j, tk = tile / tiles_j;
+        return (tk * kT + w / kT) * gj + tj * kT + w % kT;
+    }
+    return s;
+}
+
 // When cfl_ws is non-null the kernel also computes calcfl of its output
 // state (u_out with the face field bu) and writes inv_c / dt_next exactly as
 // launch_calcfl does (no clamp, no t update).
