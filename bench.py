@@ -102,6 +102,31 @@ def copy_probe(box, reps: int = 3):
     return 64.0 * box.n / s / 1e9
 
 
+def ensemble_bench(nbox: int = 96, steps: int = 50):
+    """The batched-ensemble workload: nbox periodic boxes with sides drawn by
+    seed from {32,48,64,96,128} and per-box amplitudes, autotuned launch shape,
+    graph-replayed steps.  Returns aggregate cell-updates/s and details."""
+    import torch
+    from heterogeneous_mhd_amd import solver
+    sides, specs = solver.ensemble_specs(nbox, seed=7)
+    ens = solver.Ensemble(sides, cfl=0.7)
+    ens.init(specs)
+    best_ms = ens.tune(reps=2)
+    ens.run(2, graph=True)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.run(steps, graph=True)
+    ens.sync()
+    wall = time.perf_counter() - t0
+    t, dt = ens.times()
+    out = {"boxes": nbox, "cells": ens.cells, "steps": steps, "value": ens.cells * steps / wall,
+           "unit": "cell-updates/s", "ms_per_step": 1e3 * wall / steps, "config": ens.config(),
+           "tune_best_ms": best_ms, "dt_range": [min(dt), max(dt)]}
+    ens.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +139,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=5)
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -170,6 +196,9 @@ def main():
     }
     del box
     torch.cuda.empty_cache()
+    if not args.no_ensemble:
+        print("[bench] batched ensemble", file=sys.stderr, flush=True)
+        line["ensemble"] = ensemble_bench()
     if not args.no_cpu:
         # the CPU leg steps a 256^3 box of the same generator (the GPU makes
         # the IC, bitwise equal to the host generator's)

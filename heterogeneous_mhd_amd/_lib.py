@@ -69,6 +69,22 @@ SIGNATURES = {
     "mhd_ic_generate": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_double,
                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _vp, _vp]),
     "mhd_diag_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    # batched ensembles
+    "mhd_ensemble_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(C.c_int), C.c_double,
+                                      C.c_float, _vp]),
+    "mhd_ensemble_destroy": (None, [_vp]),
+    "mhd_ensemble_sizes": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                     C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "mhd_ensemble_bind": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "mhd_ensemble_init_box": (C.c_int, [_vp, C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, _vp]),
+    "mhd_ensemble_reset_time": (C.c_int, [_vp]),
+    "mhd_ensemble_run": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mhd_ensemble_get_times": (C.c_int, [_vp, _fp, _fp]),
+    "mhd_ensemble_num_configs": (C.c_int, []),
+    "mhd_ensemble_set_config": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mhd_ensemble_get_config": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
+    "mhd_ensemble_tune": (C.c_int, [_vp, C.c_int, _fp]),
     "mhd_diagnostics": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(C.c_double),
                                   _vp]),
 }

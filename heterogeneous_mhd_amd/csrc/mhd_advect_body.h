@@ -1,0 +1,203 @@
+// mhd_advect_body.h -- advectbyzx: constrained-transport advection of the two
+// transverse face fields along the sweep axis (device code shared by the
+// single-box and the ensemble kernels).
+//
+// Reference: advectbyzx C/advectbyzx.c:14-23 (A1, A2, B1, B2 in order),
+// advectbyzxA1 C/advectbyzxA1.c:18-63, A2 C/advectbyzxA2.c:9-28,
+// B1 C/advectbyzxB1.c:19-62, B2 C/advectbyzxB2.c:9-28, tvdb C/tvdb.c:10-91.
+//
+// The reference runs four full passes and scatters the edge flux of pencil
+// (j,k) into b1 of pencil (j-1,k) (A2) and (j,k-1) (B2) through a global tmp
+// array, so that per cell
+//   b1 <- (((b1 - FA(j,k)) + FA(j+1,k)) - FB(j,k)) + FB(j,k+1)     (at i-1)
+// Here one workgroup owns pencil (j,k) and evaluates the four tvdb lines it
+// needs -- A at (j,k) and (j+1,k), B at (j,k) and (j,k+1) -- itself, so
+// there is no tmp array, no scatter, no atomics and a single pass; b1 is
+// written once in the reference's addition order.  The output b must not
+// alias update_b (neighbour pencils read update_b while this one writes b);
+// b_main is read only at the cell being written, so it may alias b.
+#pragma once
+#include "mhd_internal.h"
+
+namespace mhd {
+namespace {
+
+constexpr int kLines = 4;  // A(j), A(j+1), B(k), B(k+1)
+
+template <int BLOCK, int M>
+__device__ __forceinline__ void advect_group(float *lds, float *b_out, const float *b_main,
+                                             const float *__restrict__ uu,
+                                             const float *__restrict__ bu, int nx, int ny, int nz,
+                                             int tp, int grp, float dt)
+{
+    const int pb = BLOCK / tp;
+    const int lp = threadIdx.x / tp;
+    const int lt = threadIdx.x - lp * tp;
+    const int npen = ny * nz;
+    const int pen = grp * pb + lp;
+    const bool live = pen < npen;
+    const int pc = live ? pen : 0;             // in-range pencil for the clamped loads
+    const int j = pc % ny, k = pc / ny;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
+    const int km = k == 0 ? nz - 1 : k - 1, kp = k + 1 == nz ? 0 : k + 1;
+    const float z = 0.0f, h = 0.5f;
+
+    // three stencil arrays of kLines*nx floats per pencil, line-major
+    float *X = lds + (size_t)lp * 3 * kLines * nx;
+    float *Y = X + kLines * nx;
+    float *Z = Y + kLines * nx;
+
+    // pencil (row) offsets in cells: jm, c, jp, km, kp
+    const size_t row = (size_t)nx;
+    const size_t op[5] = {((size_t)k * ny + jm) * row, ((size_t)k * ny + j) * row,
+                          ((size_t)k * ny + jp) * row, ((size_t)km * ny + j) * row,
+                          ((size_t)kp * ny + j) * row};
+    // line t: face velocity from pencils (lo, hi); its field comes from pencil hi
+    constexpr int lo[kLines] = {0, 1, 3, 1}, hi[kLines] = {1, 2, 1, 4};
+    constexpr int comp[kLines] = {1, 1, 2, 2};
+
+    // ---- every global load first (indices clamped into range so that the
+    // loads are unconditional and all in flight together), then the raw face
+    // velocity vx = (m1(lo)+m1(hi))/(rho(lo)+rho(hi)), A1 :34 / B1 :34
+    float rho[M][5], m1[M][5];
+    float bl[M][kLines];  // the advected face field, per line
+    float b1m[M];         // main b1 of this pencil
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        const int ic = i < nx ? i : nx - 1;
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            rho[r][q] = uu[(op[q] + ic) * 5 + 0];
+            m1[r][q] = uu[(op[q] + ic) * 5 + 1];
+        }
+#pragma unroll
+        for (int t = 0; t < kLines; t++) bl[r][t] = bu[(op[hi[t]] + ic) * 3 + comp[t]];
+        b1m[r] = b_main[(op[1] + ic) * 3 + 0];
+    }
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+#pragma unroll
+            for (int t = 0; t < kLines; t++)
+                X[t * nx + i] = (m1[r][lo[t]] + m1[r][hi[t]]) / (rho[r][lo[t]] + rho[r][hi[t]]);
+        }
+    }
+    __syncthreads();
+    // ---- smoothing vx = (vx(i-1)+vx(i+1)+2vx)/4 in double, A1 :37-41
+    float vg[M][kLines];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                const float s = X[t * nx + im] + X[t * nx + ip];
+                vg[r][t] = (float)(((double)s + 2.0 * (double)X[t * nx + i]) / 4.0);
+                Y[t * nx + i] = vg[r][t];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- tvdb, C/tvdb.c:26-46: vh, and b*vg staged for the upwind pick
+    float vh[M][kLines];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int ip = i + 1 == nx ? 0 : i + 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                vh[r][t] = h * vg[r][t] + h * Y[t * nx + ip] + z;
+                X[t * nx + i] = bl[r][t] * vg[r][t];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int ip = i + 1 == nx ? 0 : i + 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++)
+                Z[t * nx + i] = vh[r][t] > 0.0f ? X[t * nx + i] : X[t * nx + ip];  // flux1
+        }
+    }
+    __syncthreads();
+    // ---- half step b1 = b - (flux1 - flux1(i-1))*dt/2, C/tvdb.c:48-52
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int im = i == 0 ? nx - 1 : i - 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                const float d = (Z[t * nx + i] - Z[t * nx + im]) * dt;
+                X[t * nx + i] = (float)((double)bl[r][t] - (double)d / 2.0);
+            }
+        }
+    }
+    __syncthreads();
+    // ---- limited flux, C/tvdb.c:54-78
+    float fx[M][kLines];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int ip = i + 1 == nx ? 0 : i + 1, ipp = ip + 1 == nx ? 0 : ip + 1;
+            const int im = i == 0 ? nx - 1 : i - 1;
+#pragma unroll
+            for (int t = 0; t < kLines; t++) {
+                const float *bb = X + t * nx, *vv = Y + t * nx;
+                float w, wp, wm;
+                if (vh[r][t] > 0.0f) {
+                    w = vv[i] * bb[i];
+                    wp = (float)((double)(vv[ip] * bb[ip] - w) / 2.0);
+                    wm = (float)((double)(w - vv[im] * bb[im]) / 2.0);
+                } else {
+                    w = vv[ip] * bb[ip];
+                    wp = (float)((double)(w - vv[ipp] * bb[ipp]) / 2.0);
+                    wm = (float)((double)(vv[i] * bb[i] - w) / 2.0);
+                }
+                float dw = 0.0f;
+                if (wm * wp > 0.0f) dw = (float)(2.0 * (double)wm * (double)wp / (double)(wm + wp));
+                fx[r][t] = (w + dw) * dt;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+#pragma unroll
+            for (int t = 0; t < kLines; t++) Z[t * nx + i] = fx[r][t];
+        }
+    }
+    __syncthreads();
+    // ---- b -= flux - flux(i-1) (C/tvdb.c:80-88); b1 gets the four edge
+    // fluxes F(i-1) in the order A1, A2, B1, B2 apply them.
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const int i = lt + r * tp;
+        if (live && i < nx) {
+            const int im = i == 0 ? nx - 1 : i - 1;
+            float fm[kLines];
+#pragma unroll
+            for (int t = 0; t < kLines; t++) fm[t] = Z[t * nx + im];
+            const float b2n = bl[r][0] - (fx[r][0] - fm[0]);
+            const float b3n = bl[r][2] - (fx[r][2] - fm[2]);
+            const float b1n = (((b1m[r] - fm[0]) + fm[1]) - fm[2]) + fm[3];
+            float *o = b_out + (op[1] + i) * 3;
+            o[0] = b1n;
+            o[1] = b2n;
+            o[2] = b3n;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace mhd

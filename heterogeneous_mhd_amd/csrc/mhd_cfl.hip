@@ -58,18 +58,6 @@ __global__ __launch_bounds__(kCB) void cfl_kernel(const float *__restrict__ u,
     cfl_block_record<kCB, kCPer>(t, off, lnan, ws, blockIdx.x);
 }
 
-// Ordered tree over the workgroup: thread t's summary precedes thread t+s's.
-__device__ __forceinline__ CflRec ordered_block_fold(CflRec acc, CflRec *sh)
-{
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 1; s < kFoldBlock; s <<= 1) {
-        if ((threadIdx.x & (2 * s - 1)) == 0) sh[threadIdx.x] = cfl_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
-        __syncthreads();
-    }
-    return sh[0];
-}
-
 __global__ __launch_bounds__(kFoldBlock) void cfl_fold_kernel(int nrec, CflOut o)
 {
     __shared__ CflRec sh[kFoldBlock];
@@ -106,9 +94,7 @@ __global__ __launch_bounds__(kFoldBlock) void cfl_fold_kernel(int nrec, CflOut o
     __syncthreads();
     r2 = ordered_block_fold(r2, sh);
     if (threadIdx.x == 0) {
-        float c;
-        if (!r2.has_nan) c = r2.after > 0.0f ? r2.after : 0.0f;          // max_2num from c = 0
-        else c = r2.after == -INFINITY ? __builtin_nanf("") : r2.after;  // NaN was the last cell
+        const float c = cfl_finish(r2);
         const float inv = 1 / c;                                         // C/calcfl.c:61
         o.inv_c[0] = inv;
         if (o.dt) {                                                      // C/mhd.c:62-64

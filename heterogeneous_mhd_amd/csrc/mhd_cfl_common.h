@@ -124,6 +124,13 @@ __device__ __forceinline__ void cfl_block_record(const float (&tval)[K], const i
     }
 }
 
+// c of C/calcfl.c from the summary of all cells (running max_2num from 0).
+__device__ __forceinline__ float cfl_finish(CflRec r)
+{
+    if (!r.has_nan) return r.after > 0.0f ? r.after : 0.0f;
+    return r.after == -INFINITY ? __builtin_nanf("") : r.after;  // NaN was the last cell
+}
+
 struct CflOut {
     void *ws;         // workspace, see above
     float *inv_c;     // 1/c
@@ -132,6 +139,19 @@ struct CflOut {
     double cfl;
     float tf;
 };
+
+// Ordered tree over the workgroup: thread t's summary precedes thread t+s's.
+// (kFoldBlock threads; sh holds kFoldBlock records.)
+__device__ __forceinline__ CflRec ordered_block_fold(CflRec acc, CflRec *sh)
+{
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 1; s < kFoldBlock; s <<= 1) {
+        if ((threadIdx.x & (2 * s - 1)) == 0) sh[threadIdx.x] = cfl_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    return sh[0];
+}
 
 // Folds the nrec level-1 records a producer launch wrote and writes the
 // result (defined in mhd_cfl.hip).

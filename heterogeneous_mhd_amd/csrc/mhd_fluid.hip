@@ -1,155 +1,8 @@
-// mhd_fluid.hip -- fluidx: the relaxing-TVD fluid sweep along the first axis.
-//
-// Reference: fluidx C/fluidx.c:19-102, tvd1 C/tvd1.c:8-118,
-// mhdflux/find_max C/mhdflux.c:9-72.
-//
-// Mapping.  One pencil (all nx cells of one (j,k)) is owned by `tp` threads
-// (a multiple of the 64-wide wavefront); a workgroup owns BLOCK/tp
-// consecutive pencils, i.e. one contiguous run of u in HBM, which it stages
-// into LDS with 16-byte loads (all issued before the first LDS store) and
-// writes back the same way.  Thread t of a pencil holds cells t, t+tp, ... in
-// registers.  Stencil neighbours go through two LDS arrays of 5 floats per
-// cell, cell-major: a stride of 5 words is odd, so a wavefront's per-cell
-// reads are free of bank conflicts.  The pencil-global freezing speed c
-// (find_max, twice per sweep) is a wavefront butterfly plus, when a pencil
-// spans several wavefronts, one LDS exchange.
-//
-// Arithmetic.  Every result is bitwise the reference's (float storage,
-// double wherever a double literal or a libm double routine appears, no FMA
-// contraction: the library is built with -ffp-contract=off).  The sweep is
-// VALU-bound, so the reference's operations are regrouped where the rounded
-// results are provably identical:
-//  * abXYc(L, R, R, c, 0, 0) = c*R + 0*R + 0 (C/includefunc_spu.h:80) is
-//    computed as c*R + (R - R): both give c*R with -0 mapped to +0 for finite
-//    R and NaN for infinite or NaN R.
-//  * dfrp(i) and dfrm(i+1) (C/tvd1.c:60-64) are the same rounded expression
-//    of fr(i+1), fr(i); likewise dflm(i+1) and dflp(i) (:86-90); and
-//    fl(i) = c*wl(i+1) (:83-84).  Each is computed once per cell and shared
-//    through LDS instead of being recomputed by three neighbours.
-//  * x / c for the pencil-constant c (C/mhdflux.c:54) is x*r corrected by one
-//    FMA residual step with r = RN(1/c); with |x| in [2^-60, 2^60] and c in
-//    [2^-30, 2^30] that is the correctly rounded quotient (Markstein's
-//    theorem for an exactly rounded reciprocal); zeros, tiny, huge and
-//    non-finite x and any c outside that range use the IEEE division.
-// The limiter 2ab/(a+b) (:78, :104) and the double square root stay IEEE.
-//
-// With a CFL workspace the kernel also computes calcfl (C/calcfl.c:13-63) of
-// the state it writes, so the last sweep of a step yields the next step's dt
-// without another pass over HBM.
-#include "mhd_cfl_common.h"
+// mhd_fluid.hip -- fluidx launchers (kernel body: mhd_fluid_body.h).
+#include "mhd_fluid_body.h"
 
 namespace mhd {
 namespace {
-
-// mhdflux for one cell (C/mhdflux.c:23-45).  Returns the cell's freezing
-// speed candidate; v receives the un-normalised flux.
-__device__ __forceinline__ float flux_cell(const float q[5], const float bc[3], float v[5])
-{
-    const float gam = (float)(5.0 / 3.0);
-    const float rho = q[0], mx = q[1], my = q[2], mz = q[3], e = q[4];
-    const float b1 = bc[0], b2 = bc[1], b3 = bc[2];
-    const float vx = mx / rho;
-    const float msq = mx * mx + my * my + mz * mz;
-    const float bsq = b1 * b1 + b2 * b2 + b3 * b3;
-    const float ps = (float)(((double)e - (double)(msq / rho) / 2.0) * ((double)gam - 1.0)
-                             + (2.0 - (double)gam) * (double)bsq / 2.0);
-    v[0] = mx;
-    v[1] = mx * vx + ps - b1 * b1;
-    v[2] = my * vx - b2 * b1;
-    v[3] = mz * vx - b3 * b1;
-    v[4] = (e + ps) * vx - b1 * (b1 * mx + b2 * my + b3 * mz) / rho;
-    const float p = (float)((double)ps - (double)bsq / 2.0);
-    return (float)(fabs((double)vx) + sqrt(fabs((double)((bsq + gam * p) / rho))));
-}
-
-// C/tvd1.c:75-79: 2ab/(a+b) in double where the float product a*b > 0.
-__device__ __forceinline__ float limiter(float a, float b)
-{
-    float r = 0.0f;
-    if (a * b > 0.0f) r = (float)(2.0 * (double)a * (double)b / (double)(a + b));
-    return r;
-}
-
-// c*R + 0*R + 0 of the reference, see the file header.
-__device__ __forceinline__ float scale0(float c, float x) { return c * x + (x - x); }
-
-// x / c, correctly rounded; r = 1/c, fast = c in [2^-30, 2^30] (uniform).
-__device__ __forceinline__ float div_c(float x, float c, float r, bool fast)
-{
-    if (!fast) return x / c;
-    const float q0 = x * r;
-    float q = fmaf(fmaf(-q0, c, x), r, q0);
-    const float ax = fabsf(x);
-    if (__builtin_expect(!(ax >= 0x1p-60f && ax <= 0x1p60f), 0)) q = (x == 0.0f) ? q0 : x / c;
-    return q;
-}
-
-// Max over the threads of one pencil.  x >= 0 and never NaN (the callers
-// start from 0 and keep the running max with a strict '>', as find_max does,
-// so a NaN candidate never enters).  Ends with a workgroup barrier, which the
-// caller relies on to recycle the LDS stencil arrays.
-template <int BLOCK>
-__device__ __forceinline__ float pencil_max(float x, int tp, float *red)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        float o = __shfl_xor(x, off, 64);
-        x = o > x ? o : x;
-    }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (tp > 64) {
-        if (lane == 0) red[wave] = x;
-        __syncthreads();
-        const int wpp = tp >> 6, first = (threadIdx.x / tp) * wpp;
-        x = red[first];
-        for (int w = 1; w < wpp; w++) {
-            float o = red[first + w];
-            x = o > x ? o : x;
-        }
-    }
-    __syncthreads();
-    return x;
-}
-
-// Copies na floats a_src -> a_dst and nb floats b_src -> b_dst (global ->
-// LDS).  With vec, 16-byte vectors: all loads of both arrays are issued
-// before the first LDS store.  Loads and stores use an index clamped into
-// range (a clamped lane rewrites the last vector with its own, identical
-// value), so there is no branch for the compiler to sink the loads into.
-template <int BLOCK, int KA, int KB>
-__device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int na, float *b_dst,
-                                         const float *b_src, int nb, bool vec)
-{
-    if (vec) {
-        const float4 *sa = reinterpret_cast<const float4 *>(a_src);
-        const float4 *sb = reinterpret_cast<const float4 *>(b_src);
-        const int na4 = na >> 2, nb4 = nb >> 2;
-        float4 ra[KA], rb[KB];
-#pragma unroll
-        for (int k = 0; k < KA; k++) {
-            const int f = threadIdx.x + k * BLOCK;
-            ra[k] = sa[f < na4 ? f : na4 - 1];
-        }
-#pragma unroll
-        for (int k = 0; k < KB; k++) {
-            const int f = threadIdx.x + k * BLOCK;
-            rb[k] = sb[f < nb4 ? f : nb4 - 1];
-        }
-#pragma unroll
-        for (int k = 0; k < KA; k++) {
-            const int f = threadIdx.x + k * BLOCK;
-            reinterpret_cast<float4 *>(a_dst)[f < na4 ? f : na4 - 1] = ra[k];
-        }
-#pragma unroll
-        for (int k = 0; k < KB; k++) {
-            const int f = threadIdx.x + k * BLOCK;
-            reinterpret_cast<float4 *>(b_dst)[f < nb4 ? f : nb4 - 1] = rb[k];
-        }
-    } else {
-        for (int f = threadIdx.x; f < na; f += BLOCK) a_dst[f] = a_src[f];
-        for (int f = threadIdx.x; f < nb; f += BLOCK) b_dst[f] = b_src[f];
-    }
-}
 
 template <int BLOCK, int M, bool CFL>
 __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float *uu,
@@ -158,238 +11,11 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float
                                                        float dtv, CflOut co)
 {
     extern __shared__ __attribute__((aligned(16))) float lds[];
-    __shared__ float red[2][BLOCK / 64];
-    const int pb = BLOCK / tp;                 // pencils per workgroup
-    const int lp = threadIdx.x / tp;           // this thread's pencil in the group
-    const int lt = threadIdx.x - lp * tp;      // this thread's rank in its pencil
-    const int npen = ny * nz;
+    __shared__ float red[2 * (BLOCK / 64)];
+    const int pb = BLOCK / tp;
     const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
-    const int pen0 = grp * pb;
-    const int npb = npen - pen0 < pb ? npen - pen0 : pb;  // pencils present
-    const int pen = pen0 + lp;
-    const bool live = lp < npb;
-    const int j = pen % ny, k = pen / ny;
-    const int jp = j + 1 == ny ? 0 : j + 1, kp = k + 1 == nz ? 0 : k + 1;  // C/fluidx.c:52-53
-
-    float *A = lds;                            // 5*nx*pb floats
-    float *B = lds + 5 * nx * pb;              // 5*nx*pb floats
-    float *Ap = A + 5 * nx * lp, *Bp = B + 5 * nx * lp;
-
     const float dt = dtp ? *dtp : dtv;
-    const float z = 0.0f, h = 0.5f;
-    constexpr int KU = (5 * M + 3) / 4, KB = (3 * M + 3) / 4;  // 16-byte loads per thread
-
-    // ---- stage the group's u (5 floats/cell) and b (3/cell) into LDS; the
-    // neighbour faces b2(j+1), b3(k+1) go straight to registers
-    const float *gu = uu + (size_t)pen0 * nx * 5;
-    const float *gb = bu + (size_t)pen0 * nx * 3;
-    const int nu = 5 * nx * npb, nb = 3 * nx * npb;
-    const bool vec = ((uintptr_t)gu % 16 == 0) && ((uintptr_t)gb % 16 == 0) && nu % 4 == 0 &&
-                     nb % 4 == 0 && (5 * nx * pb) % 4 == 0;
-    float b2jp[M], b3kp[M];
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            b2jp[r] = bu[(((size_t)k * ny + jp) * nx + i) * 3 + 1];
-            b3kp[r] = bu[(((size_t)kp * ny + j) * nx + i) * 3 + 2];
-        }
-    }
-    stage_in<BLOCK, KU, KB>(A, gu, nu, B, gb, nb, vec);
-    __syncthreads();
-
-    float q[M][5], bc[M][3], v[M][5], bf[CFL ? M : 1][3];
-    float cmax = 0.0f;
-    const float *Bs = B + 3 * nx * lp;         // b staging of this pencil (3 floats/cell)
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1;
-#pragma unroll
-            for (int c = 0; c < 5; c++) q[r][c] = Ap[5 * i + c];
-            if (CFL)
-                for (int c = 0; c < 3; c++) bf[r][c] = Bs[3 * i + c];
-            // cell-centred B, C/fluidx.c:71-81
-            const float h1 = h * Bs[3 * i], h1p = h * Bs[3 * ip];
-            bc[r][0] = h1 + h1p;
-            bc[r][1] = h * Bs[3 * i + 1] + b2jp[r] / 2;
-            bc[r][2] = h * Bs[3 * i + 2] + b3kp[r] / 2;
-            const float cf = flux_cell(q[r], bc[r], v[r]);
-            cmax = cf > cmax ? cf : cmax;
-        }
-    }
-    const float c1 = pencil_max<BLOCK>(cmax, tp, red[0]);
-
-    // ---- stage 1: half-step predictor, C/tvd1.c:36-49
-    // s[][] holds, in turn, fr of stage 1, the predictor u1, fr of stage 2 and
-    // the final flux; q keeps the sweep's input u, the base of the update (:116).
-    float s[M][5];
-    {
-        const bool norm = c1 > 0.0f;
-        const bool fast = c1 >= 0x1p-30f && c1 <= 0x1p30f;
-        const float rc = norm ? 1.0f / c1 : 0.0f;
-#pragma unroll
-        for (int r = 0; r < M; r++) {
-            const int i = lt + r * tp;
-            if (live && i < nx) {
-#pragma unroll
-                for (int c = 0; c < 5; c++) {
-                    const float vv = norm ? div_c(v[r][c], c1, rc, fast) : v[r][c];
-                    const float wr = q[r][c] + vv + z;
-                    const float wl = q[r][c] - vv + z;
-                    s[r][c] = scale0(c1, wr);          // fr
-                    Ap[5 * i + c] = scale0(c1, wl);    // fl(i-1) = c*wl(i)
-                }
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1;
-#pragma unroll
-            for (int c = 0; c < 5; c++) Bp[5 * i + c] = h * s[r][c] + (-h) * Ap[5 * ip + c] + z;  // flux
-        }
-    }
-    __syncthreads();
-    const float hdt = (float)(-1.0 * (double)dt / 2.0);
-    cmax = 0.0f;
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int im = i == 0 ? nx - 1 : i - 1;
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                const float d = Bp[5 * i + c] - Bp[5 * im + c] + z;
-                s[r][c] = q[r][c] + hdt * d + z;           // u1
-            }
-            const float cf = flux_cell(s[r], bc[r], v[r]);
-            cmax = cf > cmax ? cf : cmax;
-        }
-    }
-    const float c2 = pencil_max<BLOCK>(cmax, tp, red[1]);
-
-    // ---- stage 2: corrector with the van Leer limiter, C/tvd1.c:54-116
-    {
-        const bool norm = c2 > 0.0f;
-        const bool fast = c2 >= 0x1p-30f && c2 <= 0x1p30f;
-        const float rc = norm ? 1.0f / c2 : 0.0f;
-#pragma unroll
-        for (int r = 0; r < M; r++) {
-            const int i = lt + r * tp;
-            if (live && i < nx) {
-#pragma unroll
-                for (int c = 0; c < 5; c++) {
-                    const float vv = norm ? div_c(v[r][c], c2, rc, fast) : v[r][c];
-                    const float wr = s[r][c] + vv + z;
-                    const float wl = s[r][c] - vv + z;
-                    s[r][c] = scale0(c2, wr);          // fr
-                    Ap[5 * i + c] = s[r][c];
-                    Bp[5 * i + c] = scale0(c2, wl);    // fl(i-1)
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // D(i) = dfrm(i) = dfrp(i-1); E(i) = dflm(i) = dflp(i-1); t1 = fr - fl
-    float D[M][5], E[M][5];
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                const float fr = s[r][c], frm = Ap[5 * im + c];
-                const float flm = Bp[5 * i + c], fl0 = Bp[5 * ip + c];
-                D[r][c] = h * fr + (-h) * frm + z;
-                E[r][c] = h * flm + (-h) * fl0 + z;
-                s[r][c] = fr - fl0 + z;                // t1
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                Ap[5 * i + c] = D[r][c];
-                Bp[5 * i + c] = E[r][c];
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1;
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                const float dfr = limiter(Ap[5 * ip + c], D[r][c]);   // (dfrp, dfrm)
-                const float dfl = limiter(Bp[5 * ip + c], E[r][c]);   // (dflp, dflm)
-                const float t2 = dfr - dfl + z;
-                s[r][c] = h * s[r][c] + h * t2 + z;                   // flux
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-#pragma unroll
-            for (int c = 0; c < 5; c++) Ap[5 * i + c] = s[r][c];
-        }
-    }
-    __syncthreads();
-    const float fdt = (float)(-1.0 * (double)dt);
-    float tcf[M];
-    int toff[M];
-    int lnan = -1;
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        tcf[r] = -INFINITY;
-        toff[r] = lp * nx + i;
-        if (live && i < nx) {
-            const int im = i == 0 ? nx - 1 : i - 1;
-            float un[5];
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                const float d = Ap[5 * i + c] - Ap[5 * im + c] + z;
-                un[c] = q[r][c] + fdt * d + z;
-                Bp[5 * i + c] = un[c];
-            }
-            if (CFL) {
-                tcf[r] = cfl_cell(un, bf[r]);
-                if (tcf[r] != tcf[r]) lnan = toff[r] > lnan ? toff[r] : lnan;
-            }
-        }
-    }
-    __syncthreads();
-    {
-        float *go = u_out + (size_t)pen0 * nx * 5;
-        if (vec) {
-            const float4 *s4 = reinterpret_cast<const float4 *>(B);
-            float4 *d4 = reinterpret_cast<float4 *>(go);
-            const int n4 = nu >> 2;
-#pragma unroll
-            for (int kk = 0; kk < KU; kk++) {
-                const int f = threadIdx.x + kk * BLOCK;
-                if (f < n4) d4[f] = s4[f];
-            }
-        } else {
-            for (int f = threadIdx.x; f < nu; f += BLOCK) go[f] = B[f];
-        }
-    }
-    if (CFL) cfl_block_record<BLOCK, M>(tcf, toff, lnan, co.ws, grp);
+    fluid_group<BLOCK, M, CFL>(lds, red, u_out, uu, bu, nx, ny, nz, tp, grp, dt, co.ws, grp);
 }
 
 struct Cfg {

@@ -143,3 +143,122 @@ class Box:
     def diagnostics(self) -> dict:
         with torch.cuda.stream(self.stream):
             return diagnostics(self.u, self.b, self.nx, self.ny, self.nz)
+
+
+class Ensemble:
+    """Independent cubic periodic boxes of different sides advanced together
+    on the GPU: one persistent launch per sweep for all boxes, per-box t and
+    CFL dt, optional HIP-graph replay.  Each box evolves bitwise as a
+    :class:`Box` of the same side would."""
+
+    def __init__(self, sides, cfl: float = 0.7, tf: float = 0.0, device="cuda",
+                 stream: torch.cuda.Stream | None = None):
+        self.sides = [int(s) for s in sides]
+        self.nbox = len(self.sides)
+        self.device = torch.device(device)
+        self.stream = stream if stream is not None else torch.cuda.Stream(self.device)
+        arr = (C.c_int * self.nbox)(*self.sides)
+        h = C.c_void_p()
+        check(lib().mhd_ensemble_create(C.byref(h), self.nbox, arr, float(cfl), float(tf),
+                                        self.stream.cuda_stream), "ensemble_create")
+        self._e = h
+        uf, bf = C.c_size_t(), C.c_size_t()
+        uo = (C.c_longlong * self.nbox)()
+        bo = (C.c_longlong * self.nbox)()
+        check(lib().mhd_ensemble_sizes(h, C.byref(uf), C.byref(bf), uo, bo), "ensemble_sizes")
+        self.u_off, self.b_off = list(uo), list(bo)
+        self.u = torch.zeros(uf.value, dtype=torch.float32, device=self.device)
+        self.b = torch.zeros(bf.value, dtype=torch.float32, device=self.device)
+        self.u2 = torch.empty_like(self.u)
+        self.b2 = torch.empty_like(self.b)
+        check(lib().mhd_ensemble_bind(h, self.u.data_ptr(), self.b.data_ptr(), self.u2.data_ptr(),
+                                      self.b2.data_ptr()), "ensemble_bind")
+
+    def close(self):
+        if getattr(self, "_e", None):
+            lib().mhd_ensemble_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def cells(self) -> int:
+        return sum(n ** 3 for n in self.sides)
+
+    def box_views(self, i: int):
+        """(u, b) of box i: views into the packed device state."""
+        n = self.sides[i]
+        return (self.u[self.u_off[i]:self.u_off[i] + 5 * n ** 3],
+                self.b[self.b_off[i]:self.b_off[i] + 3 * n ** 3])
+
+    def init(self, specs):
+        """Seeded IC per box (one ICSpec per box)."""
+        nmax = max(self.sides)
+        scratch = torch.empty(3 * nmax ** 3, dtype=torch.float64, device=self.device)
+        for i, sp in enumerate(specs):
+            check(lib().mhd_ensemble_init_box(self._e, i, sp.seed, sp.amp_v, sp.amp_a, sp.rho0, sp.p0,
+                                              sp.gamma, sp.kmax, scratch.data_ptr()), "ensemble_init_box")
+        self.stream.synchronize()
+        check(lib().mhd_ensemble_reset_time(self._e), "ensemble_reset_time")
+
+    def set_box_state(self, i: int, u, b):
+        """Overwrite box i (host or device arrays); t of every box is reset."""
+        uv, bv = self.box_views(i)
+        uv.copy_(torch.as_tensor(u).reshape(-1))
+        bv.copy_(torch.as_tensor(b).reshape(-1))
+        torch.cuda.current_stream(self.device).synchronize()
+        check(lib().mhd_ensemble_reset_time(self._e), "ensemble_reset_time")
+
+    def run(self, nsteps: int, graph: bool = False):
+        check(lib().mhd_ensemble_run(self._e, int(nsteps), int(graph)), "ensemble_run")
+
+    def times(self):
+        t = (C.c_float * self.nbox)()
+        dt = (C.c_float * self.nbox)()
+        check(lib().mhd_ensemble_get_times(self._e, t, dt), "ensemble_get_times")
+        return list(t), list(dt)
+
+    def sync(self):
+        self.stream.synchronize()
+
+    def box_state(self, i: int):
+        self.sync()
+        u, b = self.box_views(i)
+        return u.cpu().numpy(), b.cpu().numpy()
+
+    def config(self) -> dict:
+        v = [C.c_int() for _ in range(5)]
+        check(lib().mhd_ensemble_get_config(self._e, *[C.byref(x) for x in v]), "ensemble_get_config")
+        return dict(zip(("shape", "grid", "block", "cells_per_thread", "threads_per_pencil"),
+                        (x.value for x in v)))
+
+    def set_config(self, shape: int, grid: int):
+        check(lib().mhd_ensemble_set_config(self._e, int(shape), int(grid)), "ensemble_set_config")
+
+    def tune(self, reps: int = 2) -> float:
+        """Autotune the launch shape and persistent grid on the current state
+        (which is restored); returns the best time per step in ms."""
+        best = C.c_float()
+        check(lib().mhd_ensemble_tune(self._e, int(reps), C.byref(best)), "ensemble_tune")
+        return best.value
+
+    def diagnostics(self, i: int) -> dict:
+        u, b = self.box_views(i)
+        n = self.sides[i]
+        with torch.cuda.stream(self.stream):
+            return diagnostics(u, b, n, n, n)
+
+
+def ensemble_specs(nbox: int, seed: int = 7, sides=(32, 48, 64, 96, 128), amp_v=(0.05, 0.4),
+                   amp_a=(0.01, 0.1)):
+    """Box sides and per-box IC specs drawn from one seed (the batched
+    ensemble workload: sides from `sides`, amplitudes uniform in the ranges)."""
+    rng = np.random.default_rng(seed)
+    s = [int(x) for x in rng.choice(np.asarray(sides), size=nbox)]
+    specs = [ICSpec(seed=int(rng.integers(1, 2 ** 31)), amp_v=float(rng.uniform(*amp_v)),
+                    amp_a=float(rng.uniform(*amp_a))) for _ in range(nbox)]
+    return s, specs

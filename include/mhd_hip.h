@@ -156,6 +156,42 @@ size_t mhd_diag_workspace_bytes(int nx, int ny, int nz);
 int mhd_diagnostics(const float *u_dev, const float *b_dev, int nx, int ny, int nz, void *workspace,
                     double *out, mhd_stream_t stream);
 
+/* ---- batched ensembles of cubic boxes (no counterpart in the reference) ---- */
+/* Boxes of sides sides[0..nbox-1] (<= 1024) advanced together: one
+ * persistent launch per sweep covers all boxes, each box keeps its own t and
+ * CFL dt, and a step can be replayed from one captured HIP graph.  Each box
+ * evolves bitwise as a single-box plan of the same size would. */
+typedef struct mhd_ensemble mhd_ensemble;
+int mhd_ensemble_create(mhd_ensemble **ens, int nbox, const int *sides, double cfl, float tf,
+                        mhd_stream_t stream);
+void mhd_ensemble_destroy(mhd_ensemble *ens);
+/* Packed state sizes (floats) and each box's float offset into them. */
+int mhd_ensemble_sizes(mhd_ensemble *ens, size_t *u_floats, size_t *b_floats, long long *u_off,
+                       long long *b_off);
+/* Binds the packed device state (u, b) and the scratch pair (u2, b2), all
+ * 16-byte aligned, of the sizes mhd_ensemble_sizes reports. */
+int mhd_ensemble_bind(mhd_ensemble *ens, float *u, float *b, float *u2, float *b2);
+/* Seeded IC of one box into the bound state (see mhd_ic_generate);
+ * scratch_dev holds 3*n^3 doubles for that box's side n. */
+int mhd_ensemble_init_box(mhd_ensemble *ens, int box, uint64_t seed, double amp_v, double amp_a,
+                          double rho0, double p0, double gamma, int kmax, double *scratch_dev);
+/* Sets every box's t to 0 (synchronous). */
+int mhd_ensemble_reset_time(mhd_ensemble *ens);
+/* nsteps steps of every box (async on the stream); use_graph replays one
+ * captured step. */
+int mhd_ensemble_run(mhd_ensemble *ens, int nsteps, int use_graph);
+/* Per-box t and dt of the last step (synchronises). */
+int mhd_ensemble_get_times(mhd_ensemble *ens, float *t, float *dt);
+/* Launch shape (threads per workgroup, cells per thread, threads per
+ * pencil) and persistent grid size.  mhd_ensemble_tune times `reps` steps of
+ * every valid candidate on the bound state, keeps the fastest and restores
+ * the state; *best_ms receives its time per step. */
+int mhd_ensemble_num_configs(void);
+int mhd_ensemble_set_config(mhd_ensemble *ens, int shape, int grid);
+int mhd_ensemble_get_config(mhd_ensemble *ens, int *shape, int *grid, int *block,
+                            int *cells_per_thread, int *threads_per_pencil);
+int mhd_ensemble_tune(mhd_ensemble *ens, int reps, float *best_ms);
+
 #ifdef __cplusplus
 }
 #endif
