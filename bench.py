@@ -84,6 +84,22 @@ def cpu_baseline(u, b, n: int, budget_s: float = 12.0):
                       f"full steps after 1 warm-up, {el:.1f} s"}
 
 
+def pmc_traffic(op: str, cells: int):
+    """HBM bytes per launch of one operator's kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same box size (counters
+    calibrated on a copy of known size), or None if there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"hbm_traffic_step_{round(cells ** (1 / 3))}.txt")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("json: "):
+            d = json.loads(ln[6:])
+            if d["cells"] == cells and op in d["ops"]:
+                return d["ops"][op]["bytes_per_launch"]
+    return None
+
+
 def copy_probe(box, reps: int = 3):
     """Achieved bandwidth of a plain device copy of the state (copy_matrix):
     64 bytes per cell moved.  Reported next to the vendor peak."""
@@ -185,7 +201,7 @@ def main():
                                f"v amp {args.amp_v}, A amp {args.amp_a}, CFL {args.cfl}",
                    "nx": n, "ny": n, "nz": n, "graph": graph},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic(dom, N),
                      "algo_bytes_per_cell": ALGO_BYTES[dom], "mean_launch_s": kt[dom]},
         "step_roofline": {"algo_bytes_per_cell_step": step_bytes / N,
                           "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
