@@ -31,7 +31,7 @@ CALLS_PER_STEP = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13":
 METRIC = "cell-updates/s & % HBM roofline per 3D step"
 
 
-def kernel_times(box, reps: int):
+def kernel_times(box, reps: int, precision: str = "exact"):
     """Mean device time (s) of each operator's launch, by HIP events recorded
     on the stream the kernels are launched on."""
     import torch
@@ -40,8 +40,8 @@ def kernel_times(box, reps: int):
     u, b, u2, b2 = box.u, box.b, box.u2, box.b2
     dt = box.dt_tensor()
     calls = {
-        "fluidx": lambda: ops.fluidx(u2, b, nx, ny, nz, dt, u, b),
-        "advectbyzx": lambda: ops.advectbyzx(u, b2, nx, ny, nz, dt, u, b),
+        "fluidx": lambda: ops.fluidx(u2, b, nx, ny, nz, dt, u, b, precision=precision),
+        "advectbyzx": lambda: ops.advectbyzx(u, b2, nx, ny, nz, dt, u, b, precision=precision),
         "transpose12": lambda: ops.transpose12(u2, b2, u, b, nx, ny, nz),
         "transpose13": lambda: ops.transpose13(u2, b2, u, b, nx, ny, nz),
         "calcfl": lambda: ops.calcfl_device(u, b, nx, ny, nz),
@@ -156,6 +156,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=5)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--precision", choices=["exact", "fast"], default="exact")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -167,7 +168,7 @@ def main():
     n = args.n
     N = n ** 3
     print(f"[bench] {n}^3 box, allocating state", file=sys.stderr, flush=True)
-    box = solver.Box(n, n, n, cfl=args.cfl)
+    box = solver.Box(n, n, n, cfl=args.cfl, precision=args.precision)
     box.init(solver.ICSpec(seed=1, amp_v=args.amp_v, amp_a=args.amp_a))
     graph = not args.no_graph
     print("[bench] IC ready, warm-up", file=sys.stderr, flush=True)
@@ -185,7 +186,7 @@ def main():
     value = N * args.steps / wall
     print(f"[bench] {ms_per_step:.2f} ms/step, t={t_sim:.4f} dt={dt:.4g}", file=sys.stderr, flush=True)
 
-    kt = kernel_times(box, args.kernel_reps)
+    kt = kernel_times(box, args.kernel_reps, args.precision)
     per_step = {k: kt[k] * CALLS_PER_STEP[k] for k in kt}
     dom = max(per_step, key=per_step.get)
     achieved = ALGO_BYTES[dom] * N / kt[dom] / 1e9
@@ -199,7 +200,7 @@ def main():
         "data": "synthetic (seeded random-phase Fourier IC generated on the GPU)",
         "config": {"workload": f"plain relaxing-TVD split MHD step, {n}^3 periodic FP32 box, "
                                f"v amp {args.amp_v}, A amp {args.amp_a}, CFL {args.cfl}",
-                   "nx": n, "ny": n, "nz": n, "graph": graph},
+                   "nx": n, "ny": n, "nz": n, "graph": graph, "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic(dom, N),
                      "algo_bytes_per_cell": ALGO_BYTES[dom], "mean_launch_s": kt[dom]},

@@ -24,6 +24,8 @@ MHD_ERR_SHAPE = -2
 MHD_ERR_HIP = -3
 MHD_ERR_TOO_LONG = -4
 MHD_ERR_NOMEM = -5
+MHD_PREC_EXACT = 0
+MHD_PREC_FAST = 1
 
 _ERRS = {MHD_ERR_ARG: "bad argument", MHD_ERR_SHAPE: "unsupported shape",
          MHD_ERR_HIP: "HIP runtime error", MHD_ERR_TOO_LONG: "pencil too long",
@@ -48,6 +50,10 @@ SIGNATURES = {
     "mhd_fluidx_async": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_float, _vp]),
     "mhd_advectbyzx_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                                        C.c_float, _vp]),
+    "mhd_fluidx_fast_async": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_float,
+                                        _vp]),
+    "mhd_advectbyzx_fast_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                            C.c_float, _vp]),
     "mhd_transpose12_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "mhd_transpose13_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "mhd_copy_matrix_async": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
@@ -63,6 +69,7 @@ SIGNATURES = {
     "mhd_plan_run": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int]),
     "mhd_plan_get_time": (C.c_int, [_vp, _fp, _fp]),
     "mhd_plan_sync": (C.c_int, [_vp]),
+    "mhd_plan_set_precision": (C.c_int, [_vp, C.c_int]),
     "mhd_ic_mode_count": (C.c_int, [C.c_int]),
     "mhd_ic_generate_host": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_double,
                                        C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
@@ -85,6 +92,7 @@ SIGNATURES = {
     "mhd_ensemble_set_config": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mhd_ensemble_get_config": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "mhd_ensemble_tune": (C.c_int, [_vp, C.c_int, _fp]),
+    "mhd_ensemble_set_precision": (C.c_int, [_vp, C.c_int]),
     "mhd_diagnostics": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(C.c_double),
                                   _vp]),
 }

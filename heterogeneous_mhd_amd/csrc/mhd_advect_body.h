@@ -16,15 +16,18 @@
 // written once in the reference's addition order.  The output b must not
 // alias update_b (neighbour pencils read update_b while this one writes b);
 // b_main is read only at the cell being written, so it may alias b.
+//
+// FAST (the tolerance mode): the same scheme in FP32 only, with the hardware
+// reciprocal (v_rcp_f32) for the face velocity and the limiter and FMA.
 #pragma once
-#include "mhd_internal.h"
+#include "mhd_fluid_body.h"  // limiter_fast
 
 namespace mhd {
 namespace {
 
 constexpr int kLines = 4;  // A(j), A(j+1), B(k), B(k+1)
 
-template <int BLOCK, int M>
+template <int BLOCK, int M, bool FAST>
 __device__ __forceinline__ void advect_group(float *lds, float *b_out, const float *b_main,
                                              const float *__restrict__ uu,
                                              const float *__restrict__ bu, int nx, int ny, int nz,
@@ -81,7 +84,9 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
         if (live && i < nx) {
 #pragma unroll
             for (int t = 0; t < kLines; t++)
-                X[t * nx + i] = (m1[r][lo[t]] + m1[r][hi[t]]) / (rho[r][lo[t]] + rho[r][hi[t]]);
+                X[t * nx + i] = FAST ? (m1[r][lo[t]] + m1[r][hi[t]]) *
+                                           __builtin_amdgcn_rcpf(rho[r][lo[t]] + rho[r][hi[t]])
+                                     : (m1[r][lo[t]] + m1[r][hi[t]]) / (rho[r][lo[t]] + rho[r][hi[t]]);
         }
     }
     __syncthreads();
@@ -95,7 +100,8 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
 #pragma unroll
             for (int t = 0; t < kLines; t++) {
                 const float s = X[t * nx + im] + X[t * nx + ip];
-                vg[r][t] = (float)(((double)s + 2.0 * (double)X[t * nx + i]) / 4.0);
+                vg[r][t] = FAST ? 0.25f * fmaf(2.0f, X[t * nx + i], s)
+                                : (float)(((double)s + 2.0 * (double)X[t * nx + i]) / 4.0);
                 Y[t * nx + i] = vg[r][t];
             }
         }
@@ -110,7 +116,7 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
             const int ip = i + 1 == nx ? 0 : i + 1;
 #pragma unroll
             for (int t = 0; t < kLines; t++) {
-                vh[r][t] = h * vg[r][t] + h * Y[t * nx + ip] + z;
+                vh[r][t] = FAST ? 0.5f * (vg[r][t] + Y[t * nx + ip]) : h * vg[r][t] + h * Y[t * nx + ip] + z;
                 X[t * nx + i] = bl[r][t] * vg[r][t];
             }
         }
@@ -135,8 +141,12 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
             const int im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
             for (int t = 0; t < kLines; t++) {
-                const float d = (Z[t * nx + i] - Z[t * nx + im]) * dt;
-                X[t * nx + i] = (float)((double)bl[r][t] - (double)d / 2.0);
+                if constexpr (FAST) {
+                    X[t * nx + i] = fmaf(-0.5f * dt, Z[t * nx + i] - Z[t * nx + im], bl[r][t]);
+                } else {
+                    const float d = (Z[t * nx + i] - Z[t * nx + im]) * dt;
+                    X[t * nx + i] = (float)((double)bl[r][t] - (double)d / 2.0);
+                }
             }
         }
     }
@@ -153,18 +163,28 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
             for (int t = 0; t < kLines; t++) {
                 const float *bb = X + t * nx, *vv = Y + t * nx;
                 float w, wp, wm;
-                if (vh[r][t] > 0.0f) {
-                    w = vv[i] * bb[i];
-                    wp = (float)((double)(vv[ip] * bb[ip] - w) / 2.0);
-                    wm = (float)((double)(w - vv[im] * bb[im]) / 2.0);
+                if constexpr (FAST) {
+                    const bool up = vh[r][t] > 0.0f;
+                    const float w0 = vv[i] * bb[i], w1 = vv[ip] * bb[ip];
+                    const float wa = up ? vv[im] * bb[im] : vv[ipp] * bb[ipp];
+                    w = up ? w0 : w1;
+                    wp = 0.5f * (up ? w1 - w0 : w1 - wa);
+                    wm = 0.5f * (up ? w0 - wa : w0 - w1);
+                    fx[r][t] = (w + limiter_fast(wm, wp)) * dt;
                 } else {
-                    w = vv[ip] * bb[ip];
-                    wp = (float)((double)(w - vv[ipp] * bb[ipp]) / 2.0);
-                    wm = (float)((double)(vv[i] * bb[i] - w) / 2.0);
+                    if (vh[r][t] > 0.0f) {
+                        w = vv[i] * bb[i];
+                        wp = (float)((double)(vv[ip] * bb[ip] - w) / 2.0);
+                        wm = (float)((double)(w - vv[im] * bb[im]) / 2.0);
+                    } else {
+                        w = vv[ip] * bb[ip];
+                        wp = (float)((double)(w - vv[ipp] * bb[ipp]) / 2.0);
+                        wm = (float)((double)(vv[i] * bb[i] - w) / 2.0);
+                    }
+                    float dw = 0.0f;
+                    if (wm * wp > 0.0f) dw = (float)(2.0 * (double)wm * (double)wp / (double)(wm + wp));
+                    fx[r][t] = (w + dw) * dt;
                 }
-                float dw = 0.0f;
-                if (wm * wp > 0.0f) dw = (float)(2.0 * (double)wm * (double)wp / (double)(wm + wp));
-                fx[r][t] = (w + dw) * dt;
             }
         }
     }

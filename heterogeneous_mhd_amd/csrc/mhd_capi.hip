@@ -266,6 +266,23 @@ int mhd_advectbyzx_async(float *b, const float *b_main, const float *update_u,
                                   (hipStream_t)stream);
 }
 
+int mhd_fluidx_fast_async(float *u, const float *update_u, const float *update_b, int nx, int ny,
+                          int nz, const float *dt_dev, float dt, mhd_stream_t stream)
+{
+    return mhd::launch_fluidx(u, update_u, update_b, nx, ny, nz, dt_dev, dt, (hipStream_t)stream,
+                              nullptr, nullptr, nullptr, 0.0, true);
+}
+
+int mhd_advectbyzx_fast_async(float *b, const float *b_main, const float *update_u,
+                              const float *update_b, int nx, int ny, int nz, const float *dt_dev,
+                              float dt, mhd_stream_t stream)
+{
+    if (b && update_b && overlaps(b, (size_t)nx * ny * nz * 12, update_b, (size_t)nx * ny * nz * 12))
+        return MHD_ERR_ARG;
+    return mhd::launch_advectbyzx(b, b_main, update_u, update_b, nx, ny, nz, dt_dev, dt,
+                                  (hipStream_t)stream, true);
+}
+
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream)
 {
@@ -316,6 +333,7 @@ struct mhd_plan {
     hipGraphExec_t exec;
     const float *gptr[4];
     const float *ready_for[2];  // state whose calcfl is in scal[2..3]
+    bool fast;                  // FP32 tolerance mode of the sweeps
 };
 
 float *mhd_plan_t_dev(mhd_plan *p) { return p ? p->scal : nullptr; }
@@ -350,6 +368,7 @@ int mhd_plan_create(mhd_plan **plan, int nx, int ny, int nz, double cfl, float t
 void mhd_plan_destroy(mhd_plan *p)
 {
     if (!p) return;
+    (void)hipStreamSynchronize(p->s);  // queued steps still use the plan's scalars and workspace
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->scal) (void)hipFree(p->scal);
@@ -367,6 +386,20 @@ int mhd_plan_set_time(mhd_plan *p, float t)
     return e == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
+int mhd_plan_set_precision(mhd_plan *p, int mode)
+{
+    if (!p || (mode != MHD_PREC_EXACT && mode != MHD_PREC_FAST)) return MHD_ERR_ARG;
+    if (p->fast != (mode == MHD_PREC_FAST) && p->exec) {
+        (void)hipStreamSynchronize(p->s);  // the graph may still be running
+        (void)hipGraphExecDestroy(p->exec);
+        (void)hipGraphDestroy(p->graph);
+        p->exec = nullptr;
+        p->graph = nullptr;
+    }
+    p->fast = mode == MHD_PREC_FAST;
+    return MHD_OK;
+}
+
 // One step, C/mhd.c:59-124.  Operators map state -> state; the reference's
 // copy_matrix calls disappear into a ping-pong between (U0,B0) and (U1,B1).
 // fluidx works in place; advectbyzx and the transposes write the other
@@ -379,6 +412,7 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
     const int nx = p->nx, ny = p->ny, nz = p->nz;
     const hipStream_t s = p->s;
     float *t = p->scal, *dt = p->scal + 1, *inv = p->scal + 2, *dtn = p->scal + 3;
+    const bool f = p->fast;
     int rc = MHD_OK;
 #define RUN(x)                    \
     do {                          \
@@ -386,26 +420,26 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
     } while (0)
     RUN(mhd::launch_begin_step(t, dt, dtn, p->tf, s));                                  // :63-64
     // x sweep
-    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s));                       // :71
-    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, nx, ny, nz, dt, 0.0f, s));               // :74
+    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :71
+    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, nx, ny, nz, dt, 0.0f, s, f));               // :74
     RUN(mhd::launch_transpose12(U1, B0, U0, B1, nx, ny, nz, s));                         // :78
     // y sweep, dims (ny, nx, nz)
-    RUN(mhd::launch_fluidx(U1, U1, B0, ny, nx, nz, dt, 0.0f, s));                       // :80
-    RUN(mhd::launch_advectbyzx(B1, B0, U1, B0, ny, nx, nz, dt, 0.0f, s));               // :84
+    RUN(mhd::launch_fluidx(U1, U1, B0, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :80
+    RUN(mhd::launch_advectbyzx(B1, B0, U1, B0, ny, nx, nz, dt, 0.0f, s, f));               // :84
     RUN(mhd::launch_transpose13(U0, B0, U1, B1, ny, nx, nz, s));                         // :88
     // z sweeps, dims (nz, nx, ny)
-    RUN(mhd::launch_fluidx(U0, U0, B0, nz, nx, ny, dt, 0.0f, s));                       // :91
-    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, nz, nx, ny, dt, 0.0f, s));               // :93
-    RUN(mhd::launch_advectbyzx(B0, B1, U0, B1, nz, nx, ny, dt, 0.0f, s));               // :95
-    RUN(mhd::launch_fluidx(U0, U0, B0, nz, nx, ny, dt, 0.0f, s));                       // :97
+    RUN(mhd::launch_fluidx(U0, U0, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :91
+    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, nz, nx, ny, dt, 0.0f, s, f));               // :93
+    RUN(mhd::launch_advectbyzx(B0, B1, U0, B1, nz, nx, ny, dt, 0.0f, s, f));               // :95
+    RUN(mhd::launch_fluidx(U0, U0, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :97
     RUN(mhd::launch_transpose13(U1, B1, U0, B0, nz, nx, ny, s));                         // :102
     // y again, dims (ny, nx, nz)
-    RUN(mhd::launch_advectbyzx(B0, B1, U1, B1, ny, nx, nz, dt, 0.0f, s));               // :105
-    RUN(mhd::launch_fluidx(U1, U1, B0, ny, nx, nz, dt, 0.0f, s));                       // :107
+    RUN(mhd::launch_advectbyzx(B0, B1, U1, B1, ny, nx, nz, dt, 0.0f, s, f));               // :105
+    RUN(mhd::launch_fluidx(U1, U1, B0, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :107
     RUN(mhd::launch_transpose12(U0, B1, U1, B0, ny, nx, nz, s));                         // :112
     // x again; its sweep also reduces calcfl of the new state for the next step
-    RUN(mhd::launch_advectbyzx(B0, B1, U0, B1, nx, ny, nz, dt, 0.0f, s));               // :115
-    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl)); // :117
+    RUN(mhd::launch_advectbyzx(B0, B1, U0, B1, nx, ny, nz, dt, 0.0f, s, f));               // :115
+    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl, f));  // :117
 #undef RUN
     return rc;
 }
@@ -430,6 +464,7 @@ int mhd_plan_run(mhd_plan *p, float *u, float *b, float *u2, float *b2, int nste
     }
     const float *want[4] = {u, b, u2, b2};
     if (p->exec && memcmp(want, p->gptr, sizeof(want)) != 0) {
+        (void)hipStreamSynchronize(p->s);  // the old graph may still be running
         (void)hipGraphExecDestroy(p->exec);
         (void)hipGraphDestroy(p->graph);
         p->exec = nullptr;

@@ -56,7 +56,7 @@ __device__ __forceinline__ int find_box(const int *pre, int nbox, int item)
     return lo;
 }
 
-template <int BLOCK, int M, bool CFL>
+template <int BLOCK, int M, bool CFL, bool FAST>
 __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *U, const float *Bf,
                                                           int tp, void *ws)
 {
@@ -69,12 +69,12 @@ __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *U, co
         float *u = U + E.uoff[box];
         __syncthreads();  // the previous item's last LDS reads are done
         // record index = item: each box's records are contiguous, in memory order
-        fluid_group<BLOCK, M, CFL>(lds, red, u, u, Bf + E.boff[box], n, n, n, tp, item - E.gpre[box],
-                                   E.dt[box], ws, item);
+        fluid_group<BLOCK, M, CFL, FAST>(lds, red, u, u, Bf + E.boff[box], n, n, n, tp,
+                                         item - E.gpre[box], E.dt[box], ws, item);
     }
 }
 
-template <int BLOCK, int M>
+template <int BLOCK, int M, bool FAST>
 __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout, const float *Bmain,
                                                            const float *U, const float *Bin, int tp)
 {
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout
         const int n = E.n[box];
         const long long bo = E.boff[box];
         __syncthreads();
-        advect_group<BLOCK, M>(lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n, n, tp,
+        advect_group<BLOCK, M, FAST>(lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n, n, tp,
                                item - E.gpre[box], E.dt[box]);
     }
 }
@@ -164,66 +164,80 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kGridMult[] = {2, 4, 8, 16};  // persistent workgroups per CU
 constexpr int kNumGrids = sizeof(kGridMult) / sizeof(kGridMult[0]);
 
+template <int BLOCK, int M, bool CFL, bool FAST>
+int launch_fluid_k(const EnsDev &E, float *U, const float *B, int tp, size_t lds, void *ws, int grid,
+                   hipStream_t s)
+{
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, CFL, FAST>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
+                       U, B, tp, ws);
+    return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
 template <int BLOCK, int M>
 int launch_fluid_shape(const EnsDev &E, float *U, const float *B, int tp, int nmax, bool cfl,
-                       void *ws, int grid, hipStream_t s)
+                       bool fast, void *ws, int grid, hipStream_t s)
 {
     const size_t lds = sizeof(float) * 10 * (size_t)nmax * (BLOCK / tp);
+    if (cfl)
+        return fast ? launch_fluid_k<BLOCK, M, true, true>(E, U, B, tp, lds, ws, grid, s)
+                    : launch_fluid_k<BLOCK, M, true, false>(E, U, B, tp, lds, ws, grid, s);
+    return fast ? launch_fluid_k<BLOCK, M, false, true>(E, U, B, tp, lds, ws, grid, s)
+                : launch_fluid_k<BLOCK, M, false, false>(E, U, B, tp, lds, ws, grid, s);
+}
+
+template <int BLOCK, int M, bool FAST>
+int launch_advect_k(const EnsDev &E, float *Bo, const float *Bm, const float *U, const float *Bi,
+                    int tp, size_t lds, int grid, hipStream_t s)
+{
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)ens_advect_kernel<BLOCK, M, FAST>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
-    if (cfl) {
-        if (lds > 65536)
-            (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, true>), dim3(grid), dim3(BLOCK), lds, s, E, U, B, tp, ws);
-    } else {
-        if (lds > 65536)
-            (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, false>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, false>), dim3(grid), dim3(BLOCK), lds, s, E, U, B, tp, ws);
-    }
+    hipLaunchKernelGGL((ens_advect_kernel<BLOCK, M, FAST>), dim3(grid), dim3(BLOCK), lds, s, E, Bo,
+                       Bm, U, Bi, tp);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
 template <int BLOCK, int M>
 int launch_advect_shape(const EnsDev &E, float *Bo, const float *Bm, const float *U, const float *Bi,
-                        int tp, int nmax, int grid, hipStream_t s)
+                        int tp, int nmax, bool fast, int grid, hipStream_t s)
 {
     const size_t lds = sizeof(float) * 12 * (size_t)nmax * (BLOCK / tp);
-    if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)ens_advect_kernel<BLOCK, M>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipGetLastError();
-    hipLaunchKernelGGL((ens_advect_kernel<BLOCK, M>), dim3(grid), dim3(BLOCK), lds, s, E, Bo, Bm, U, Bi, tp);
-    return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+    return fast ? launch_advect_k<BLOCK, M, true>(E, Bo, Bm, U, Bi, tp, lds, grid, s)
+                : launch_advect_k<BLOCK, M, false>(E, Bo, Bm, U, Bi, tp, lds, grid, s);
 }
 
-int launch_fluid(int shape, const EnsDev &E, float *U, const float *B, int nmax, bool cfl, void *ws,
-                 int grid, hipStream_t s)
+int launch_fluid(int shape, bool fast, const EnsDev &E, float *U, const float *B, int nmax, bool cfl,
+                 void *ws, int grid, hipStream_t s)
 {
     const Shape &c = kShapes[shape];
     switch (shape) {
-    case 0: return launch_fluid_shape<256, 2>(E, U, B, c.tp, nmax, cfl, ws, grid, s);
-    case 1: return launch_fluid_shape<128, 2>(E, U, B, c.tp, nmax, cfl, ws, grid, s);
-    case 2: return launch_fluid_shape<256, 1>(E, U, B, c.tp, nmax, cfl, ws, grid, s);
-    case 3: return launch_fluid_shape<512, 2>(E, U, B, c.tp, nmax, cfl, ws, grid, s);
-    case 4: return launch_fluid_shape<256, 2>(E, U, B, c.tp, nmax, cfl, ws, grid, s);
-    case 5: return launch_fluid_shape<512, 2>(E, U, B, c.tp, nmax, cfl, ws, grid, s);
-    default: return launch_fluid_shape<1024, 2>(E, U, B, c.tp, nmax, cfl, ws, grid, s);
+    case 0: return launch_fluid_shape<256, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
+    case 1: return launch_fluid_shape<128, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
+    case 2: return launch_fluid_shape<256, 1>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
+    case 3: return launch_fluid_shape<512, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
+    case 4: return launch_fluid_shape<256, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
+    case 5: return launch_fluid_shape<512, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
+    default: return launch_fluid_shape<1024, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
     }
 }
 
-int launch_advect(int shape, const EnsDev &E, float *Bo, const float *Bm, const float *U,
+int launch_advect(int shape, bool fast, const EnsDev &E, float *Bo, const float *Bm, const float *U,
                   const float *Bi, int nmax, int grid, hipStream_t s)
 {
     const Shape &c = kShapes[shape];
     switch (shape) {
-    case 0: return launch_advect_shape<256, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, grid, s);
-    case 1: return launch_advect_shape<128, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, grid, s);
-    case 2: return launch_advect_shape<256, 1>(E, Bo, Bm, U, Bi, c.tp, nmax, grid, s);
-    case 3: return launch_advect_shape<512, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, grid, s);
-    case 4: return launch_advect_shape<256, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, grid, s);
-    case 5: return launch_advect_shape<512, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, grid, s);
-    default: return launch_advect_shape<1024, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, grid, s);
+    case 0: return launch_advect_shape<256, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
+    case 1: return launch_advect_shape<128, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
+    case 2: return launch_advect_shape<256, 1>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
+    case 3: return launch_advect_shape<512, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
+    case 4: return launch_advect_shape<256, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
+    case 5: return launch_advect_shape<512, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
+    default: return launch_advect_shape<1024, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
     }
 }
 
@@ -265,6 +279,7 @@ struct mhd_ensemble {
     std::vector<long long> uoff, boff;  // per box (caller's order)
     size_t ufloats, bfloats;
     int shape, grid;                    // current launch shape and persistent grid size
+    bool fast;                          // FP32 tolerance mode of the sweeps
     // device tables
     void *tab;
     EnsDev E;
@@ -281,6 +296,7 @@ int ens_error(hipError_t e) { return e == hipSuccess ? MHD_OK : MHD_ERR_HIP; }
 
 void drop_graph(mhd_ensemble *e)
 {
+    if (e->exec) (void)hipStreamSynchronize(e->s);  // the graph may still be running
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->exec = nullptr;
@@ -293,6 +309,7 @@ int ens_step(mhd_ensemble *e)
     const hipStream_t s = e->s;
     float *U0 = e->U0, *B0 = e->B0, *U1 = e->U1, *B1 = e->B1;
     const int sh = e->shape, g = e->grid, nm = e->nmax;
+    const bool f = e->fast;
     int rc = MHD_OK;
 #define RUN(x)                      \
     do {                            \
@@ -301,22 +318,22 @@ int ens_step(mhd_ensemble *e)
     (void)hipGetLastError();
     hipLaunchKernelGGL(ens_begin_kernel, dim3((e->nbox + 63) / 64), dim3(64), 0, s, E, e->tf);  // :63-64
     RUN(ens_error(hipGetLastError()));
-    RUN(launch_fluid(sh, E, U0, B0, nm, false, nullptr, g, s));            // :71
-    RUN(launch_advect(sh, E, B1, B0, U0, B0, nm, g, s));                   // :74
+    RUN(launch_fluid(sh, f, E, U0, B0, nm, false, nullptr, g, s));            // :71
+    RUN(launch_advect(sh, f, E, B1, B0, U0, B0, nm, g, s));                   // :74
     RUN(launch_transpose(E, U1, B0, U0, B1, false, g, s));                 // :78
-    RUN(launch_fluid(sh, E, U1, B0, nm, false, nullptr, g, s));            // :80
-    RUN(launch_advect(sh, E, B1, B0, U1, B0, nm, g, s));                   // :84
+    RUN(launch_fluid(sh, f, E, U1, B0, nm, false, nullptr, g, s));            // :80
+    RUN(launch_advect(sh, f, E, B1, B0, U1, B0, nm, g, s));                   // :84
     RUN(launch_transpose(E, U0, B0, U1, B1, true, g, s));                  // :88
-    RUN(launch_fluid(sh, E, U0, B0, nm, false, nullptr, g, s));            // :91
-    RUN(launch_advect(sh, E, B1, B0, U0, B0, nm, g, s));                   // :93
-    RUN(launch_advect(sh, E, B0, B1, U0, B1, nm, g, s));                   // :95
-    RUN(launch_fluid(sh, E, U0, B0, nm, false, nullptr, g, s));            // :97
+    RUN(launch_fluid(sh, f, E, U0, B0, nm, false, nullptr, g, s));            // :91
+    RUN(launch_advect(sh, f, E, B1, B0, U0, B0, nm, g, s));                   // :93
+    RUN(launch_advect(sh, f, E, B0, B1, U0, B1, nm, g, s));                   // :95
+    RUN(launch_fluid(sh, f, E, U0, B0, nm, false, nullptr, g, s));            // :97
     RUN(launch_transpose(E, U1, B1, U0, B0, true, g, s));                  // :102
-    RUN(launch_advect(sh, E, B0, B1, U1, B1, nm, g, s));                   // :105
-    RUN(launch_fluid(sh, E, U1, B0, nm, false, nullptr, g, s));            // :107
+    RUN(launch_advect(sh, f, E, B0, B1, U1, B1, nm, g, s));                   // :105
+    RUN(launch_fluid(sh, f, E, U1, B0, nm, false, nullptr, g, s));            // :107
     RUN(launch_transpose(E, U0, B1, U1, B0, false, g, s));                 // :112
-    RUN(launch_advect(sh, E, B0, B1, U0, B1, nm, g, s));                   // :115
-    RUN(launch_fluid(sh, E, U0, B0, nm, true, e->ws, g, s));               // :117 + next calcfl
+    RUN(launch_advect(sh, f, E, B0, B1, U0, B1, nm, g, s));                   // :115
+    RUN(launch_fluid(sh, f, E, U0, B0, nm, true, e->ws, g, s));               // :117 + next calcfl
     if (rc == MHD_OK) {
         hipLaunchKernelGGL(ens_cfl_fold_kernel, dim3(e->nbox), dim3(kFoldBlock), 0, s, E, e->ws, e->cfl);
         rc = ens_error(hipGetLastError());
@@ -455,6 +472,7 @@ int mhd_ensemble_create(mhd_ensemble **out, int nbox, const int *sides, double c
 void mhd_ensemble_destroy(mhd_ensemble *e)
 {
     if (!e) return;
+    (void)hipStreamSynchronize(e->s);  // queued steps still use the tables and workspace
     drop_graph(e);
     if (e->tab) (void)hipFree(e->tab);
     if (e->ws) (void)hipFree(e->ws);
@@ -490,6 +508,14 @@ int mhd_ensemble_set_config(mhd_ensemble *e, int shape, int grid)
     e->shape = shape;
     e->grid = grid;
     drop_graph(e);
+    return MHD_OK;
+}
+
+int mhd_ensemble_set_precision(mhd_ensemble *e, int mode)
+{
+    if (!e || (mode != MHD_PREC_EXACT && mode != MHD_PREC_FAST)) return MHD_ERR_ARG;
+    if (e->fast != (mode == MHD_PREC_FAST)) drop_graph(e);
+    e->fast = mode == MHD_PREC_FAST;
     return MHD_OK;
 }
 

@@ -4,7 +4,7 @@
 namespace mhd {
 namespace {
 
-template <int BLOCK, int M, bool CFL>
+template <int BLOCK, int M, bool CFL, bool FAST>
 __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float *uu,
                                                        const float *__restrict__ bu, int nx, int ny,
                                                        int nz, int tp, const float *__restrict__ dtp,
@@ -15,7 +15,7 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float
     const int pb = BLOCK / tp;
     const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
     const float dt = dtp ? *dtp : dtv;
-    fluid_group<BLOCK, M, CFL>(lds, red, u_out, uu, bu, nx, ny, nz, tp, grp, dt, co.ws, grp);
+    fluid_group<BLOCK, M, CFL, FAST>(lds, red, u_out, uu, bu, nx, ny, nz, tp, grp, dt, co.ws, grp);
 }
 
 struct Cfg {
@@ -32,7 +32,7 @@ Cfg pick(int nx)
     return {1024, 2, 1024};
 }
 
-template <int BLOCK, int M, bool CFL>
+template <int BLOCK, int M, bool CFL, bool FAST>
 int launch_cfg(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz, int tp,
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
 {
@@ -41,23 +41,34 @@ int launch_cfg(float *u_out, const float *uu, const float *bu, int nx, int ny, i
     const long nblk = (npen + pb - 1) / pb;
     const size_t lds = sizeof(float) * 10 * (size_t)nx * pb;
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)fluidx_kernel<BLOCK, M, CFL>,
+        (void)hipFuncSetAttribute((const void *)fluidx_kernel<BLOCK, M, CFL, FAST>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
-    hipLaunchKernelGGL((fluidx_kernel<BLOCK, M, CFL>), dim3((unsigned)nblk), dim3(BLOCK), lds, s,
+    hipLaunchKernelGGL((fluidx_kernel<BLOCK, M, CFL, FAST>), dim3((unsigned)nblk), dim3(BLOCK), lds, s,
                        u_out, uu, bu, nx, ny, nz, tp, dtp, dtv, co);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
-template <bool CFL>
+template <bool CFL, bool FAST>
 int launch_any(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
 {
     const Cfg c = pick(nx);
-    if (c.block == 256 && c.m == 1) return launch_cfg<256, 1, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
-    if (c.block == 256) return launch_cfg<256, 2, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
-    if (c.block == 512) return launch_cfg<512, 2, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
-    return launch_cfg<1024, 2, CFL>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    if (c.block == 256 && c.m == 1)
+        return launch_cfg<256, 1, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    if (c.block == 256)
+        return launch_cfg<256, 2, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    if (c.block == 512)
+        return launch_cfg<512, 2, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    return launch_cfg<1024, 2, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+}
+
+template <bool CFL>
+int launch_mode(bool fast, float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
+                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
+{
+    return fast ? launch_any<CFL, true>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s)
+                : launch_any<CFL, false>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
 }
 
 }  // namespace
@@ -71,7 +82,7 @@ size_t fluid_workgroups(int nx, long npen)
 
 int launch_fluidx(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
                   const float *dt_dev, float dt_val, hipStream_t s, void *cfl_ws, float *inv_c,
-                  float *dt_next, double cfl)
+                  float *dt_next, double cfl, bool fast)
 {
     if (!u_out || !uu || !bu || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
     if (nx > kMaxPencil) return MHD_ERR_TOO_LONG;
@@ -79,11 +90,11 @@ int launch_fluidx(float *u_out, const float *uu, const float *bu, int nx, int ny
     CflOut co = {cfl_ws, inv_c, dt_next, nullptr, cfl, 0.0f};
     if (cfl_ws) {
         if (!inv_c) return MHD_ERR_ARG;
-        const int rc = launch_any<true>(u_out, uu, bu, nx, ny, nz, dt_dev, dt_val, co, s);
+        const int rc = launch_mode<true>(fast, u_out, uu, bu, nx, ny, nz, dt_dev, dt_val, co, s);
         if (rc != MHD_OK) return rc;
         return launch_cfl_fold((int)fluid_workgroups(nx, (long)ny * nz), co, s);
     }
-    return launch_any<false>(u_out, uu, bu, nx, ny, nz, dt_dev, dt_val, co, s);
+    return launch_mode<false>(fast, u_out, uu, bu, nx, ny, nz, dt_dev, dt_val, co, s);
 }
 
 namespace {

@@ -34,6 +34,12 @@
 //    non-finite x and any c outside that range use the IEEE division.
 // The limiter 2ab/(a+b) (:78, :104) and the double square root stay IEEE.
 //
+// FAST (the tolerance mode) keeps the scheme and its operation order but
+// computes in FP32 only: hardware reciprocal and square root (v_rcp_f32,
+// v_sqrt_f32, 1 ulp), FMA, and c*(w +- v/c) evaluated as c*w +- v without the
+// division.  Its results differ from the reference's by FP32 rounding noise;
+// the tests state the tolerance.
+//
 // With a CFL workspace the kernel also computes calcfl (C/calcfl.c:13-63) of
 // the state it writes, so the last sweep of a step yields the next step's dt
 // without another pass over HBM.
@@ -62,6 +68,34 @@ __device__ __forceinline__ float flux_cell(const float q[5], const float bc[3], 
     v[4] = (e + ps) * vx - b1 * (b1 * mx + b2 * my + b3 * mz) / rho;
     const float p = (float)((double)ps - (double)bsq / 2.0);
     return (float)(fabs((double)vx) + sqrt(fabs((double)((bsq + gam * p) / rho))));
+}
+
+// flux_cell in FP32 with hardware reciprocal / square root (tolerance mode).
+__device__ __forceinline__ float flux_cell_fast(const float q[5], const float bc[3], float v[5])
+{
+    const float gam = 5.0f / 3.0f;
+    const float rho = q[0], mx = q[1], my = q[2], mz = q[3], e = q[4];
+    const float b1 = bc[0], b2 = bc[1], b3 = bc[2];
+    const float rr = __builtin_amdgcn_rcpf(rho);
+    const float vx = mx * rr;
+    const float msq = fmaf(mx, mx, fmaf(my, my, mz * mz));
+    const float bsq = fmaf(b1, b1, fmaf(b2, b2, b3 * b3));
+    const float ps = fmaf(fmaf(-0.5f * msq, rr, e), gam - 1.0f, (1.0f - 0.5f * gam) * bsq);
+    v[0] = mx;
+    v[1] = fmaf(-b1, b1, fmaf(mx, vx, ps));
+    v[2] = fmaf(my, vx, -b2 * b1);
+    v[3] = fmaf(mz, vx, -b3 * b1);
+    const float bm = fmaf(b1, mx, fmaf(b2, my, b3 * mz));
+    v[4] = fmaf(e + ps, vx, -b1 * bm * rr);
+    const float p = fmaf(-0.5f, bsq, ps);
+    return fabsf(vx) + __builtin_amdgcn_sqrtf(fabsf(fmaf(gam, p, bsq) * rr));
+}
+
+// 2ab/(a+b) where a*b > 0, FP32 with the hardware reciprocal.
+__device__ __forceinline__ float limiter_fast(float a, float b)
+{
+    const float p = a * b;
+    return p > 0.0f ? 2.0f * p * __builtin_amdgcn_rcpf(a + b) : 0.0f;
 }
 
 // C/tvd1.c:75-79: 2ab/(a+b) in double where the float product a*b > 0.
@@ -153,7 +187,7 @@ __device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int n
     }
 }
 
-template <int BLOCK, int M, bool CFL>
+template <int BLOCK, int M, bool CFL, bool FAST>
 __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out, const float *uu,
                                             const float *__restrict__ bu, int nx, int ny, int nz,
                                             int tp, int grp, float dt, void *cfl_ws, int rec)
@@ -212,7 +246,7 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
             bc[r][0] = h1 + h1p;
             bc[r][1] = h * Bs[3 * i + 1] + b2jp[r] / 2;
             bc[r][2] = h * Bs[3 * i + 2] + b3kp[r] / 2;
-            const float cf = flux_cell(q[r], bc[r], v[r]);
+            const float cf = FAST ? flux_cell_fast(q[r], bc[r], v[r]) : flux_cell(q[r], bc[r], v[r]);
             cmax = cf > cmax ? cf : cmax;
         }
     }
@@ -232,11 +266,22 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
             if (live && i < nx) {
 #pragma unroll
                 for (int c = 0; c < 5; c++) {
-                    const float vv = norm ? div_c(v[r][c], c1, rc, fast) : v[r][c];
-                    const float wr = q[r][c] + vv + z;
-                    const float wl = q[r][c] - vv + z;
-                    s[r][c] = scale0(c1, wr);          // fr
-                    Ap[5 * i + c] = scale0(c1, wl);    // fl(i-1) = c*wl(i)
+                    if constexpr (FAST) {
+                        // norm is uniform over the pencil: a branch, not a select
+                        if (norm) {
+                            s[r][c] = fmaf(c1, q[r][c], v[r][c]);
+                            Ap[5 * i + c] = fmaf(c1, q[r][c], -v[r][c]);
+                        } else {
+                            s[r][c] = 0.0f;
+                            Ap[5 * i + c] = 0.0f;
+                        }
+                    } else {
+                        const float vv = norm ? div_c(v[r][c], c1, rc, fast) : v[r][c];
+                        const float wr = q[r][c] + vv + z;
+                        const float wl = q[r][c] - vv + z;
+                        s[r][c] = scale0(c1, wr);          // fr
+                        Ap[5 * i + c] = scale0(c1, wl);    // fl(i-1) = c*wl(i)
+                    }
                 }
             }
         }
@@ -248,7 +293,9 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
         if (live && i < nx) {
             const int ip = i + 1 == nx ? 0 : i + 1;
 #pragma unroll
-            for (int c = 0; c < 5; c++) Bp[5 * i + c] = h * s[r][c] + (-h) * Ap[5 * ip + c] + z;  // flux
+            for (int c = 0; c < 5; c++)  // flux
+                Bp[5 * i + c] = FAST ? 0.5f * (s[r][c] - Ap[5 * ip + c])
+                                     : h * s[r][c] + (-h) * Ap[5 * ip + c] + z;
         }
     }
     __syncthreads();
@@ -261,10 +308,14 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
             const int im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
             for (int c = 0; c < 5; c++) {
-                const float d = Bp[5 * i + c] - Bp[5 * im + c] + z;
-                s[r][c] = q[r][c] + hdt * d + z;           // u1
+                if constexpr (FAST) {
+                    s[r][c] = fmaf(hdt, Bp[5 * i + c] - Bp[5 * im + c], q[r][c]);
+                } else {
+                    const float d = Bp[5 * i + c] - Bp[5 * im + c] + z;
+                    s[r][c] = q[r][c] + hdt * d + z;           // u1
+                }
             }
-            const float cf = flux_cell(s[r], bc[r], v[r]);
+            const float cf = FAST ? flux_cell_fast(s[r], bc[r], v[r]) : flux_cell(s[r], bc[r], v[r]);
             cmax = cf > cmax ? cf : cmax;
         }
     }
@@ -281,12 +332,24 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
             if (live && i < nx) {
 #pragma unroll
                 for (int c = 0; c < 5; c++) {
-                    const float vv = norm ? div_c(v[r][c], c2, rc, fast) : v[r][c];
-                    const float wr = s[r][c] + vv + z;
-                    const float wl = s[r][c] - vv + z;
-                    s[r][c] = scale0(c2, wr);          // fr
-                    Ap[5 * i + c] = s[r][c];
-                    Bp[5 * i + c] = scale0(c2, wl);    // fl(i-1)
+                    if constexpr (FAST) {
+                        const float u1 = s[r][c];
+                        if (norm) {
+                            s[r][c] = fmaf(c2, u1, v[r][c]);
+                            Bp[5 * i + c] = fmaf(c2, u1, -v[r][c]);
+                        } else {
+                            s[r][c] = 0.0f;
+                            Bp[5 * i + c] = 0.0f;
+                        }
+                        Ap[5 * i + c] = s[r][c];
+                    } else {
+                        const float vv = norm ? div_c(v[r][c], c2, rc, fast) : v[r][c];
+                        const float wr = s[r][c] + vv + z;
+                        const float wl = s[r][c] - vv + z;
+                        s[r][c] = scale0(c2, wr);          // fr
+                        Ap[5 * i + c] = s[r][c];
+                        Bp[5 * i + c] = scale0(c2, wl);    // fl(i-1)
+                    }
                 }
             }
         }
@@ -303,9 +366,15 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
             for (int c = 0; c < 5; c++) {
                 const float fr = s[r][c], frm = Ap[5 * im + c];
                 const float flm = Bp[5 * i + c], fl0 = Bp[5 * ip + c];
-                D[r][c] = h * fr + (-h) * frm + z;
-                E[r][c] = h * flm + (-h) * fl0 + z;
-                s[r][c] = fr - fl0 + z;                // t1
+                if constexpr (FAST) {
+                    D[r][c] = 0.5f * (fr - frm);
+                    E[r][c] = 0.5f * (flm - fl0);
+                    s[r][c] = fr - fl0;
+                } else {
+                    D[r][c] = h * fr + (-h) * frm + z;
+                    E[r][c] = h * flm + (-h) * fl0 + z;
+                    s[r][c] = fr - fl0 + z;                // t1
+                }
             }
         }
     }
@@ -329,10 +398,16 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
             const int ip = i + 1 == nx ? 0 : i + 1;
 #pragma unroll
             for (int c = 0; c < 5; c++) {
-                const float dfr = limiter(Ap[5 * ip + c], D[r][c]);   // (dfrp, dfrm)
-                const float dfl = limiter(Bp[5 * ip + c], E[r][c]);   // (dflp, dflm)
-                const float t2 = dfr - dfl + z;
-                s[r][c] = h * s[r][c] + h * t2 + z;                   // flux
+                if constexpr (FAST) {
+                    const float dfr = limiter_fast(Ap[5 * ip + c], D[r][c]);
+                    const float dfl = limiter_fast(Bp[5 * ip + c], E[r][c]);
+                    s[r][c] = 0.5f * (s[r][c] + (dfr - dfl));
+                } else {
+                    const float dfr = limiter(Ap[5 * ip + c], D[r][c]);   // (dfrp, dfrm)
+                    const float dfl = limiter(Bp[5 * ip + c], E[r][c]);   // (dflp, dflm)
+                    const float t2 = dfr - dfl + z;
+                    s[r][c] = h * s[r][c] + h * t2 + z;                   // flux
+                }
             }
         }
     }
@@ -360,8 +435,12 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
             float un[5];
 #pragma unroll
             for (int c = 0; c < 5; c++) {
-                const float d = Ap[5 * i + c] - Ap[5 * im + c] + z;
-                un[c] = q[r][c] + fdt * d + z;
+                if constexpr (FAST) {
+                    un[c] = fmaf(fdt, Ap[5 * i + c] - Ap[5 * im + c], q[r][c]);
+                } else {
+                    const float d = Ap[5 * i + c] - Ap[5 * im + c] + z;
+                    un[c] = q[r][c] + fdt * d + z;
+                }
                 Bp[5 * i + c] = un[c];
             }
             if (CFL) {

@@ -48,18 +48,23 @@ __device__ __forceinline__ int sweep_group(int bid, int G, int gj, int nz)
     return s;
 }
 
+// fast selects the FP32 tolerance mode of the sweep kernels (hardware
+// reciprocal and square root, FMA); otherwise results are bitwise the
+// reference's.
 // When cfl_ws is non-null the kernel also computes calcfl of its output
 // state (u_out with the face field bu) and writes inv_c / dt_next exactly as
 // launch_calcfl does (no clamp, no t update).
 int launch_fluidx(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
                   const float *dt_dev, float dt_val, hipStream_t s, void *cfl_ws = nullptr,
-                  float *inv_c = nullptr, float *dt_next = nullptr, double cfl = 0.0);
+                  float *inv_c = nullptr, float *dt_next = nullptr, double cfl = 0.0,
+                  bool fast = false);
 
 // dt = dt_next, clamped to (tf - t)/2 when tf > 0; t += 2 dt  (C/mhd.c:63-64)
 int launch_begin_step(float *t, float *dt, const float *dt_next, float tf, hipStream_t s);
 
 int launch_advectbyzx(float *b_out, const float *b_main, const float *uu, const float *bu, int nx,
-                      int ny, int nz, const float *dt_dev, float dt_val, hipStream_t s);
+                      int ny, int nz, const float *dt_dev, float dt_val, hipStream_t s,
+                      bool fast = false);
 
 int launch_transpose12(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                        int nz, hipStream_t s);

@@ -39,17 +39,18 @@ def _dt(dt):
     return None, float(dt)
 
 
-def fluidx(u, b, nx, ny, nz, dt, update_u, update_b, iter=0):
-    """Fluid sweep along the first axis (C/fluidx.c:19-102); writes u."""
+def fluidx(u, b, nx, ny, nz, dt, update_u, update_b, iter=0, precision="exact"):
+    """Fluid sweep along the first axis (C/fluidx.c:19-102); writes u.
+    precision 'fast' selects the FP32 tolerance mode."""
     del b, iter
     n = nx * ny * nz
     dp, dv = _dt(dt)
-    check(lib().mhd_fluidx_async(_dev(u, 5 * n, "u"), _dev(update_u, 5 * n, "update_u"),
-                                 _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv, _stream()),
-          "fluidx")
+    f = lib().mhd_fluidx_fast_async if precision == "fast" else lib().mhd_fluidx_async
+    check(f(_dev(u, 5 * n, "u"), _dev(update_u, 5 * n, "update_u"),
+            _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv, _stream()), "fluidx")
 
 
-def advectbyzx(u, b, nx, ny, nz, dt, update_u, update_b):
+def advectbyzx(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
     """CT advection of b2, b3 along the first axis (C/advectbyzx.c:14-23).
 
     Reads update_u, update_b and main b(1); writes b.  b must not alias
@@ -59,9 +60,9 @@ def advectbyzx(u, b, nx, ny, nz, dt, update_u, update_b):
     n = nx * ny * nz
     dp, dv = _dt(dt)
     pb = _dev(b, 3 * n, "b")
-    check(lib().mhd_advectbyzx_async(pb, pb, _dev(update_u, 5 * n, "update_u"),
-                                     _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv,
-                                     _stream()), "advectbyzx")
+    f = lib().mhd_advectbyzx_fast_async if precision == "fast" else lib().mhd_advectbyzx_async
+    check(f(pb, pb, _dev(update_u, 5 * n, "update_u"), _dev(update_b, 3 * n, "update_b"), nx, ny,
+            nz, dp, dv, _stream()), "advectbyzx")
 
 
 def transpose12(u, b, update_u, update_b, nx, ny, nz, iter=0):

@@ -26,9 +26,12 @@ import sys
 
 # kernel-name prefix -> operator whose C call launches it
 _OPS = [
-    (re.compile(r"fluidx_kernel<\d+, \d+, false>"), "fluidx"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, true>"), "fluidx+calcfl (fused, last sweep)"),
-    (re.compile(r"advect_kernel<"), "advectbyzx"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, false, false>"), "fluidx"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, false, true>"), "fluidx fast"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, true, false>"), "fluidx+calcfl (fused, last sweep)"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, true, true>"), "fluidx+calcfl fast (fused, last sweep)"),
+    (re.compile(r"advect_kernel<\d+, \d+, false>"), "advectbyzx"),
+    (re.compile(r"advect_kernel<\d+, \d+, true>"), "advectbyzx fast"),
     (re.compile(r"transpose_kernel<5>"), "transpose (u, 5 comps)"),
     (re.compile(r"transpose_kernel<3>"), "transpose (b, 3 comps)"),
     (re.compile(r"cfl_kernel"), "calcfl"),

@@ -15,7 +15,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from ._lib import MHDError, check, lib
+from ._lib import MHD_PREC_EXACT, MHD_PREC_FAST, MHDError, check, lib
 
 GAMMA = 5.0 / 3.0
 
@@ -71,11 +71,18 @@ def diagnostics(u: torch.Tensor, b: torch.Tensor, nx: int, ny: int, nz: int) -> 
             "divb_rms": v[6]}
 
 
+def _prec(precision: str) -> int:
+    modes = {"exact": MHD_PREC_EXACT, "fast": MHD_PREC_FAST}
+    if precision not in modes:
+        raise MHDError(f"precision must be one of {sorted(modes)}, not {precision!r}")
+    return modes[precision]
+
+
 class Box:
     """One periodic box advanced on the GPU by the native step driver."""
 
     def __init__(self, nx: int, ny: int, nz: int, cfl: float = 0.7, tf: float = 0.0,
-                 device="cuda", stream: torch.cuda.Stream | None = None):
+                 device="cuda", stream: torch.cuda.Stream | None = None, precision: str = "exact"):
         if ny != nz:
             raise MHDError("the split step needs ny == nz (transpose13 swaps y and z)")
         self.nx, self.ny, self.nz = nx, ny, nz
@@ -92,6 +99,12 @@ class Box:
         check(lib().mhd_plan_create(C.byref(h), nx, ny, nz, float(cfl), float(tf),
                                     self.stream.cuda_stream), "plan_create")
         self._plan = h
+        self.set_precision(precision)
+
+    def set_precision(self, precision: str):
+        """'exact' (bitwise the reference) or 'fast' (FP32 tolerance mode)."""
+        check(lib().mhd_plan_set_precision(self._plan, _prec(precision)), "plan_set_precision")
+        self.precision = precision
 
     def close(self):
         if getattr(self, "_plan", None):
@@ -106,6 +119,7 @@ class Box:
 
     # -- state -------------------------------------------------------------
     def set_state(self, u, b, t: float = 0.0):
+        self.stream.synchronize()  # steps still queued on the box's stream use the old state
         self.u.copy_(torch.as_tensor(u).reshape(-1))
         self.b.copy_(torch.as_tensor(b).reshape(-1))
         torch.cuda.current_stream(self.device).synchronize()
@@ -152,7 +166,7 @@ class Ensemble:
     :class:`Box` of the same side would."""
 
     def __init__(self, sides, cfl: float = 0.7, tf: float = 0.0, device="cuda",
-                 stream: torch.cuda.Stream | None = None):
+                 stream: torch.cuda.Stream | None = None, precision: str = "exact"):
         self.sides = [int(s) for s in sides]
         self.nbox = len(self.sides)
         self.device = torch.device(device)
@@ -173,6 +187,12 @@ class Ensemble:
         self.b2 = torch.empty_like(self.b)
         check(lib().mhd_ensemble_bind(h, self.u.data_ptr(), self.b.data_ptr(), self.u2.data_ptr(),
                                       self.b2.data_ptr()), "ensemble_bind")
+        self.set_precision(precision)
+
+    def set_precision(self, precision: str):
+        """'exact' (bitwise the reference) or 'fast' (FP32 tolerance mode)."""
+        check(lib().mhd_ensemble_set_precision(self._e, _prec(precision)), "ensemble_set_precision")
+        self.precision = precision
 
     def close(self):
         if getattr(self, "_e", None):
@@ -208,6 +228,7 @@ class Ensemble:
     def set_box_state(self, i: int, u, b):
         """Overwrite box i (host or device arrays); t of every box is reset."""
         uv, bv = self.box_views(i)
+        self.stream.synchronize()  # steps still queued on the ensemble's stream
         uv.copy_(torch.as_tensor(u).reshape(-1))
         bv.copy_(torch.as_tensor(b).reshape(-1))
         torch.cuda.current_stream(self.device).synchronize()

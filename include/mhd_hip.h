@@ -74,6 +74,15 @@ float calcfl(float *u, float *b, int *nx, int *ny, int *nz);
 
 typedef void *mhd_stream_t; /* a hipStream_t; NULL is the null stream */
 
+/* Arithmetic of the sweep kernels in plans and ensembles.  EXACT (the
+ * default) reproduces the reference bitwise.  FAST keeps the scheme and
+ * computes in FP32 only (hardware reciprocal and square root, FMA); its
+ * fields differ from the reference's by FP32 rounding noise, within the
+ * relative tolerance the library's tests state.  Transposes and calcfl are
+ * exact in both modes. */
+#define MHD_PREC_EXACT 0
+#define MHD_PREC_FAST 1
+
 /* Library version string and the longest pencil the sweeps accept. */
 const char *mhd_version(void);
 int mhd_max_pencil(void);
@@ -89,6 +98,13 @@ int mhd_fluidx_async(float *u, const float *update_u, const float *update_b, int
 int mhd_advectbyzx_async(float *b, const float *b_main, const float *update_u,
                          const float *update_b, int nx, int ny, int nz, const float *dt_dev,
                          float dt, mhd_stream_t stream);
+/* The same two sweeps in MHD_PREC_FAST arithmetic (no reference counterpart:
+ * the reference has only the exact form). */
+int mhd_fluidx_fast_async(float *u, const float *update_u, const float *update_b, int nx, int ny,
+                          int nz, const float *dt_dev, float dt, mhd_stream_t stream);
+int mhd_advectbyzx_fast_async(float *b, const float *b_main, const float *update_u,
+                              const float *update_b, int nx, int ny, int nz, const float *dt_dev,
+                              float dt, mhd_stream_t stream);
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream);
 int mhd_transpose13_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
@@ -127,6 +143,8 @@ int mhd_plan_set_time(mhd_plan *plan, float t);
  * step into a HIP graph on first use and replays it.  Asynchronous on the
  * plan's stream. */
 int mhd_plan_run(mhd_plan *plan, float *u, float *b, float *u2, float *b2, int nsteps, int use_graph);
+/* Selects MHD_PREC_EXACT or MHD_PREC_FAST for the plan's later steps. */
+int mhd_plan_set_precision(mhd_plan *plan, int mode);
 /* Copies t and dt to the host (synchronises the plan's stream). */
 int mhd_plan_get_time(mhd_plan *plan, float *t, float *dt);
 /* Synchronises the plan's stream. */
@@ -191,6 +209,8 @@ int mhd_ensemble_set_config(mhd_ensemble *ens, int shape, int grid);
 int mhd_ensemble_get_config(mhd_ensemble *ens, int *shape, int *grid, int *block,
                             int *cells_per_thread, int *threads_per_pencil);
 int mhd_ensemble_tune(mhd_ensemble *ens, int reps, float *best_ms);
+/* Selects MHD_PREC_EXACT or MHD_PREC_FAST for the ensemble's later steps. */
+int mhd_ensemble_set_precision(mhd_ensemble *ens, int mode);
 
 #ifdef __cplusplus
 }
