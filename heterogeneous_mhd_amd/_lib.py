@@ -93,6 +93,24 @@ SIGNATURES = {
     "mhd_ensemble_get_config": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "mhd_ensemble_tune": (C.c_int, [_vp, C.c_int, _fp]),
     "mhd_ensemble_set_precision": (C.c_int, [_vp, C.c_int]),
+    # learned corrector
+    "mhd_plan_set_tf": (C.c_int, [_vp, C.c_float]),
+    "mhd_plan_state_changed": (C.c_int, [_vp]),
+    "mhd_corr_num_params": (C.c_int, []),
+    "mhd_corr_conv": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "mhd_corr_dw_workspace_bytes": (C.c_size_t, []),
+    "mhd_corr_conv_dw": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp]),
+    "mhd_corr_dw_reduce": (C.c_int, [_vp, _vp, _vp]),
+    "mhd_corr_weight_prep": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mhd_corr_adam": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_float, C.c_float, C.c_float,
+                                C.c_float, C.c_int, C.c_float, _vp]),
+    "mhd_corr_pack": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "mhd_corr_loss_partials": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "mhd_corr_loss_grad": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "mhd_downsample": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "mhd_corr_check_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "mhd_corr_apply": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_double, _vp, _vp,
+                                 _vp, _vp, _vp]),
     "mhd_diagnostics": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(C.c_double),
                                   _vp]),
 }

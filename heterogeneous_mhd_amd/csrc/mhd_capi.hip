@@ -386,6 +386,27 @@ int mhd_plan_set_time(mhd_plan *p, float t)
     return e == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
+int mhd_plan_set_tf(mhd_plan *p, float tf)
+{
+    if (!p) return MHD_ERR_ARG;
+    if (p->tf != tf && p->exec) {
+        (void)hipStreamSynchronize(p->s);  // tf is baked into the captured graph
+        (void)hipGraphExecDestroy(p->exec);
+        (void)hipGraphDestroy(p->graph);
+        p->exec = nullptr;
+        p->graph = nullptr;
+    }
+    p->tf = tf;
+    return MHD_OK;
+}
+
+int mhd_plan_state_changed(mhd_plan *p)
+{
+    if (!p) return MHD_ERR_ARG;
+    p->ready_for[0] = p->ready_for[1] = nullptr;
+    return MHD_OK;
+}
+
 int mhd_plan_set_precision(mhd_plan *p, int mode)
 {
     if (!p || (mode != MHD_PREC_EXACT && mode != MHD_PREC_FAST)) return MHD_ERR_ARG;

@@ -1,0 +1,691 @@
+// mhd_corrector.hip -- the learned coarse-grid corrector: a 4-layer 3x3x3
+// periodic convolution network (32 channels, GELU, residual output) trained
+// and run on MI355X with hand-written BF16 MFMA kernels accumulating in FP32,
+// plus the downsampling that makes its training data and the fallback checks
+// that guard its use.  No counterpart in the reference.
+//
+// Layouts.  A voxel index is v = (k*ny + j)*nx + i (the state's cell order).
+// Activations are bf16 [V][32], channel fastest, so the 32 input channels of
+// one voxel are one 64-byte run; the 8 physical channels (rho, m1..m3, E,
+// cell-centred B1..B3) are zero-padded to 32 so every layer is 32 -> 32.
+// Convolution weights are bf16 [27][32 out][32 in], tap t = (dz+1)*9 +
+// (dy+1)*3 + (dx+1) for the neighbour offset (dx,dy,dz); boundaries are
+// periodic, as the box is.
+//
+// Forward and input-gradient kernel (implicit GEMM).  A wavefront owns 64
+// consecutive voxels of one x-row (M) and all 32 output channels (N).  Each
+// of the 27 taps is one K-step of 32 input channels: the A operand (16
+// voxels x 32 channels) is one 16-byte load per lane straight from HBM/L2,
+// the B operand (32 channels x 16 outputs) one 16-byte LDS read per lane from
+// the weights staged once per workgroup; v_mfma_f32_16x16x32_bf16 does
+// the product, 8 per tap (4 voxel blocks x 2 output blocks).  The input
+// gradient is the same convolution with taps mirrored and the weight matrix
+// transposed (mhd_corr_weight_prep writes both forms).  Epilogues: bias +
+// GELU (storing the FP32 pre-activation for the backward pass), bias only
+// (output layer), or multiplication by GELU'(z) of the previous layer (the
+// backward pass through the activation).
+//
+// Weight gradient.  dW[t][o][c] = sum_v dY[v][o] * X[v + off_t][c] is a GEMM
+// whose K is the voxel index.  A workgroup walks x-rows; for each it stages
+// dY of the row and X of the 9 neighbour rows (with one halo voxel each
+// side) channel-major in LDS, so each lane's 8 consecutive voxels are
+// contiguous; each wavefront owns one 16x16 (o, c) block for all 27 taps.
+// Partial sums are written per workgroup and folded in a fixed order, so
+// the gradient is deterministic.
+#include "mhd_internal.h"
+#include "mhd_hip.h"
+
+#include <math.h>
+
+namespace mhd {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+constexpr int kC = 32;           // channels of every layer
+constexpr int kTaps = 27;
+constexpr int kWElems = kTaps * kC * kC;
+constexpr int kParams = kWElems + kC;  // weights + bias of one layer
+constexpr int kDwGroups = 256;   // workgroups of the weight-gradient kernel
+
+__device__ __forceinline__ u16 f2bf(float x)
+{
+    unsigned u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                        // nearest even
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// GELU in its erfc form, 0.5 z erfc(-z/sqrt2): no cancellation for z < 0
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * erfcf(-z * 0.70710678118654752f); }
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    return 0.5f * erfcf(-z * 0.70710678118654752f) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+
+enum { kModeHidden = 0, kModeOutput = 1, kModeGrad = 2 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
+                                                   const float *__restrict__ bias, int nx, int ny,
+                                                   int nz, float *__restrict__ zout,
+                                                   u16 *__restrict__ aout,
+                                                   const float *__restrict__ zprev)
+{
+    __shared__ __attribute__((aligned(16))) u16 wl[kWElems];
+    for (int f = threadIdx.x; f < kWElems / 8; f += 256)
+        reinterpret_cast<uint4 *>(wl)[f] = reinterpret_cast<const uint4 *>(W)[f];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lm = lane & 15, lg = lane >> 4;
+    const long v0 = ((long)blockIdx.x * 4 + wave) * 64;  // this wavefront's 64 voxels
+    const int i0 = (int)(v0 % nx);
+    const long row = v0 / nx;
+    const int j = (int)(row % ny), k = (int)(row / ny);
+
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int t = 0; t < kTaps; t++) {
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        const long rb = ((long)wrap(k + dz, nz) * ny + wrap(j + dy, ny)) * nx;
+        bf16x8 bw[2];
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++)
+            bw[nb] = *reinterpret_cast<const bf16x8 *>(wl + (t * kC + 16 * nb + lm) * kC + 8 * lg);
+        bf16x8 av[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            const int ii = wrap(i0 + 16 * mb + lm + dx, nx);
+            av[mb] = *reinterpret_cast<const bf16x8 *>(X + (rb + ii) * kC + 8 * lg);
+        }
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[mb], bw[nb], acc[mb][nb], 0, 0, 0);
+    }
+    // lane holds rows (voxels) 16*mb + 4*lg + r, column (channel) 16*nb + lm
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int o = 16 * nb + lm;
+        const float bo = MODE == kModeGrad ? 0.0f : bias[o];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long v = v0 + 16 * mb + 4 * lg + r;
+                const float x = acc[mb][nb][r];
+                if (MODE == kModeHidden) {
+                    const float z = x + bo;
+                    zout[v * kC + o] = z;
+                    aout[v * kC + o] = f2bf(gelu(z));
+                } else if (MODE == kModeOutput) {
+                    zout[v * kC + o] = x + bo;
+                } else {
+                    aout[v * kC + o] = f2bf(x * gelu_grad(zprev[v * kC + o]));
+                }
+            }
+    }
+}
+
+// Weight gradient, see the file header.  ws[g] (kParams floats per
+// workgroup g) receives (or, with accumulate, adds) the group's partial dW
+// and bias gradient.
+__global__ __launch_bounds__(256) void conv_dw_kernel(const u16 *__restrict__ dY,
+                                                      const u16 *__restrict__ X, int nx, int ny,
+                                                      int nz, float *__restrict__ ws, int accumulate)
+{
+    extern __shared__ __attribute__((aligned(16))) u16 sm[];
+    const int wn = nx + 2;                 // staged row length with halo
+    u16 *dyt = sm;                         // [32][nx]
+    u16 *xt = sm + kC * nx;                // [9][32][nx + 2]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lm = lane & 15, lg = lane >> 4;
+    const int ob = wave >> 1, cb = wave & 1;  // this wavefront's (o, c) block
+    const long nrow = (long)ny * nz;
+    f32x4 acc[kTaps];
+#pragma unroll
+    for (int t = 0; t < kTaps; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float db = 0.0f;  // bias gradient of channel threadIdx.x (< 32)
+
+    for (long row = blockIdx.x; row < nrow; row += gridDim.x) {
+        const int j = (int)(row % ny), k = (int)(row / ny);
+        __syncthreads();  // the previous row's LDS reads are done
+        // dY of the row, channel-major
+        for (int f = threadIdx.x; f < nx * 4; f += 256) {
+            const int i = f >> 2, c8 = (f & 3) * 8;
+            const uint4 q = *reinterpret_cast<const uint4 *>(dY + (row * nx + i) * kC + c8);
+            const u16 *h = reinterpret_cast<const u16 *>(&q);
+#pragma unroll
+            for (int e = 0; e < 8; e++) dyt[(c8 + e) * nx + i] = h[e];
+        }
+        // X of the 9 neighbour rows, voxels -1..nx
+        for (int f = threadIdx.x; f < 9 * wn * 4; f += 256) {
+            const int q9 = f / (wn * 4), rem = f - q9 * wn * 4;
+            const int ih = rem >> 2, c8 = (rem & 3) * 8;
+            const int dz = q9 / 3 - 1, dy = q9 % 3 - 1;
+            const long src = ((long)wrap(k + dz, nz) * ny + wrap(j + dy, ny)) * nx + wrap(ih - 1, nx);
+            const uint4 q = *reinterpret_cast<const uint4 *>(X + src * kC + c8);
+            const u16 *h = reinterpret_cast<const u16 *>(&q);
+#pragma unroll
+            for (int e = 0; e < 8; e++) xt[(q9 * kC + c8 + e) * wn + ih] = h[e];
+        }
+        __syncthreads();
+        if (threadIdx.x < kC)
+            for (int i = 0; i < nx; i++) db += bf2f(dyt[threadIdx.x * nx + i]);
+        for (int kc = 0; kc < nx / 32; kc++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(dyt + (16 * ob + lm) * nx + 32 * kc + 8 * lg);
+#pragma unroll
+            for (int t = 0; t < kTaps; t++) {
+                const int q9 = t / 3, dx = t % 3 - 1;
+                const u16 *p = xt + (q9 * kC + 16 * cb + lm) * wn + 32 * kc + 8 * lg + dx + 1;
+                bf16x8 b;
+#pragma unroll
+                for (int e = 0; e < 8; e++) b[e] = __builtin_bit_cast(__bf16, p[e]);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    float *out = ws + (size_t)blockIdx.x * kParams;
+#pragma unroll
+    for (int t = 0; t < kTaps; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int o = 16 * ob + 4 * lg + r, c = 16 * cb + lm;
+            float *d = out + (t * kC + o) * kC + c;
+            *d = accumulate ? *d + acc[t][r] : acc[t][r];
+        }
+    if (threadIdx.x < kC) {
+        float *d = out + kWElems + threadIdx.x;
+        *d = accumulate ? *d + db : db;
+    }
+}
+
+// grad[e] += sum over the workgroups' partials, in workgroup order
+__global__ void dw_reduce_kernel(const float *__restrict__ ws, int groups, float *__restrict__ grad)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kParams) return;
+    float s = 0.0f;
+    for (int g = 0; g < groups; g++) s += ws[(size_t)g * kParams + e];
+    grad[e] += s;
+}
+
+// FP32 master weights [27][32 o][32 c] + bias -> bf16 forward form
+// [t][o][c] and input-gradient form [26 - t][c][o].
+__global__ void weight_prep_kernel(const float *__restrict__ w, u16 *__restrict__ wf, u16 *__restrict__ wb)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kWElems) return;
+    const int t = e / (kC * kC), o = (e / kC) % kC, c = e % kC;
+    const u16 h = f2bf(w[e]);
+    wf[e] = h;
+    wb[((kTaps - 1 - t) * kC + c) * kC + o] = h;
+}
+
+__global__ void adam_kernel(float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, int n, float lr, float b1, float b2, float eps,
+                            float c1, float c2, float gscale)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float gr = g[e] * gscale;
+    const float mm = b1 * m[e] + (1.0f - b1) * gr;
+    const float vv = b2 * v[e] + (1.0f - b2) * gr * gr;
+    m[e] = mm;
+    v[e] = vv;
+    p[e] -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+    g[e] = 0.0f;
+}
+
+__device__ __forceinline__ long vid(int i, int j, int k, int nx, int ny)
+{
+    return ((long)k * ny + j) * nx + i;
+}
+
+// state (u AoS 5, b faces 3) -> normalised bf16 input [V][32]
+__global__ void pack_kernel(const float *__restrict__ u, const float *__restrict__ b, int nx, int ny,
+                            int nz, const float *__restrict__ norm, u16 *__restrict__ x)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)nx * ny * nz) return;
+    const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
+    float f[8];
+#pragma unroll
+    for (int c = 0; c < 5; c++) f[c] = u[v * 5 + c];
+    f[5] = 0.5f * (b[v * 3 + 0] + b[vid(wrap(i + 1, nx), j, k, nx, ny) * 3 + 0]);
+    f[6] = 0.5f * (b[v * 3 + 1] + b[vid(i, wrap(j + 1, ny), k, nx, ny) * 3 + 1]);
+    f[7] = 0.5f * (b[v * 3 + 2] + b[vid(i, j, wrap(k + 1, nz), nx, ny) * 3 + 2]);
+    u16 h[kC];
+#pragma unroll
+    for (int c = 0; c < 8; c++) h[c] = f2bf((f[c] - norm[c]) * norm[8 + c]);
+#pragma unroll
+    for (int c = 8; c < kC; c++) h[c] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        reinterpret_cast<uint4 *>(x + v * kC)[q] = reinterpret_cast<const uint4 *>(h)[q];
+}
+
+// b = curl A with A on edges (A1 at (i, j-1/2, k-1/2), ...), forward differences;
+// A is read from channels 5..7 of a [V][ld] array.
+__device__ __forceinline__ void curl_at(const float *A, int ld, int i, int j, int k, int nx, int ny,
+                                        int nz, float out[3])
+{
+    const float *a0 = A + vid(i, j, k, nx, ny) * ld + 5;
+    const float *ai = A + vid(wrap(i + 1, nx), j, k, nx, ny) * ld + 5;
+    const float *aj = A + vid(i, wrap(j + 1, ny), k, nx, ny) * ld + 5;
+    const float *ak = A + vid(i, j, wrap(k + 1, nz), nx, ny) * ld + 5;
+    out[0] = (aj[2] - a0[2]) - (ak[1] - a0[1]);
+    out[1] = (ak[0] - a0[0]) - (ai[2] - a0[2]);
+    out[2] = (ai[1] - a0[1]) - (aj[0] - a0[0]);
+}
+
+// R[v][0..4] = y - target (u part); R[v][5..7] = curl(y A-channels) - target
+// (B part); lossp[block] = sum of R^2 over the block (double).
+__global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__ y,
+                                                       const float *__restrict__ tgt, int nx, int ny,
+                                                       int nz, float *__restrict__ R,
+                                                       double *__restrict__ lossp)
+{
+    __shared__ double red[256];
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (v < (long)nx * ny * nz) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            const float r = y[v * kC + c] - tgt[v * 8 + c];
+            R[v * 8 + c] = r;
+            s += (double)r * r;
+        }
+        float cb[3];
+        curl_at(y, kC, i, j, k, nx, ny, nz, cb);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const float r = cb[d] - tgt[v * 8 + 5 + d];
+            R[v * 8 + 5 + d] = r;
+            s += (double)r * r;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lossp[blockIdx.x] = red[0];
+}
+
+// dY = d(mean over voxels of sum R^2)/dy: 2R/V on the u channels, the curl
+// adjoint of 2R_B/V on the A channels, 0 on the padding channels.
+__global__ void loss_grad_kernel(const float *__restrict__ R, int nx, int ny, int nz, float gscale,
+                                 u16 *__restrict__ dy)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)nx * ny * nz) return;
+    const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
+    const float *r0 = R + v * 8 + 5;
+    const float *rim = R + vid(wrap(i - 1, nx), j, k, nx, ny) * 8 + 5;
+    const float *rjm = R + vid(i, wrap(j - 1, ny), k, nx, ny) * 8 + 5;
+    const float *rkm = R + vid(i, j, wrap(k - 1, nz), nx, ny) * 8 + 5;
+    float g[8];
+#pragma unroll
+    for (int c = 0; c < 5; c++) g[c] = gscale * R[v * 8 + c];
+    g[5] = gscale * ((rkm[1] - r0[1]) - (rjm[2] - r0[2]));
+    g[6] = gscale * ((r0[0] - rkm[0]) + (rim[2] - r0[2]));
+    g[7] = gscale * ((rjm[0] - r0[0]) - (rim[1] - r0[1]));
+    u16 h[kC];
+#pragma unroll
+    for (int c = 0; c < 8; c++) h[c] = f2bf(g[c]);
+#pragma unroll
+    for (int c = 8; c < kC; c++) h[c] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        reinterpret_cast<uint4 *>(dy + v * kC)[q] = reinterpret_cast<const uint4 *>(h)[q];
+}
+
+// Block averages of a fine state (factor r): u over r^3 cells, each face
+// component over the r^2 fine faces on the coarse face (keeps div b = 0).
+__global__ void downsample_kernel(const float *__restrict__ u, const float *__restrict__ b, int nx,
+                                  int ny, int nz, int r, float *__restrict__ uc, float *__restrict__ bc)
+{
+    const int cx = nx / r, cy = ny / r, cz = nz / r;
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)cx * cy * cz) return;
+    const int I = (int)(v % cx), J = (int)((v / cx) % cy), K = (int)(v / ((long)cx * cy));
+    double su[5] = {0, 0, 0, 0, 0}, sb[3] = {0, 0, 0};
+    for (int c = 0; c < r; c++)
+        for (int bq = 0; bq < r; bq++) {
+            for (int a = 0; a < r; a++) {
+                const float *q = u + vid(r * I + a, r * J + bq, r * K + c, nx, ny) * 5;
+#pragma unroll
+                for (int m = 0; m < 5; m++) su[m] += q[m];
+            }
+            sb[0] += b[vid(r * I, r * J + bq, r * K + c, nx, ny) * 3 + 0];
+            sb[1] += b[vid(r * I + bq, r * J, r * K + c, nx, ny) * 3 + 1];
+            sb[2] += b[vid(r * I + bq, r * J + c, r * K, nx, ny) * 3 + 2];
+        }
+    const double iv = 1.0 / ((double)r * r * r), ia = 1.0 / ((double)r * r);
+#pragma unroll
+    for (int m = 0; m < 5; m++) uc[v * 5 + m] = (float)(su[m] * iv);
+#pragma unroll
+    for (int d = 0; d < 3; d++) bc[v * 3 + d] = (float)(sb[d] * ia);
+}
+
+// Per block: sums of the u channels of the correction (double).
+__global__ __launch_bounds__(256) void ysum_kernel(const float *__restrict__ y, long n,
+                                                   double *__restrict__ part)
+{
+    __shared__ double red[5][256];
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 5; c++) red[c][threadIdx.x] = v < n ? (double)y[v * kC + c] : 0.0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int c = 0; c < 5; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[5 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One workgroup: the box means of the u channels, folded in block order.
+__global__ __launch_bounds__(256) void ymean_kernel(const double *__restrict__ part, int nblk, long n,
+                                                    float *__restrict__ mean)
+{
+    __shared__ double red[5][256];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        double a = 0.0;
+        for (int q = threadIdx.x; q < nblk; q += 256) a += part[5 * q + c];
+        red[c][threadIdx.x] = a;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int c = 0; c < 5; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) mean[threadIdx.x] = (float)(red[threadIdx.x][0] / (double)n);
+}
+
+// candidate = state + correction: u += s_c * (y_c - mean y_c) (c < 5; the
+// scheme conserves the box totals of u and so does the fine solution, so the
+// correction is projected to zero total), b += s_B * curl(y_A)
+__global__ void apply_kernel(const float *__restrict__ u, const float *__restrict__ b,
+                             const float *__restrict__ y, int nx, int ny, int nz,
+                             const float *__restrict__ scale, const float *__restrict__ ymean,
+                             float *__restrict__ uo, float *__restrict__ bo)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)nx * ny * nz) return;
+    const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
+#pragma unroll
+    for (int c = 0; c < 5; c++) uo[v * 5 + c] = u[v * 5 + c] + scale[c] * (y[v * kC + c] - ymean[c]);
+    float cb[3];
+    curl_at(y, kC, i, j, k, nx, ny, nz, cb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) bo[v * 3 + d] = b[v * 3 + d] + scale[5] * cb[d];
+}
+
+// Per block: sums of E of the state and of the candidate (double), and
+// whether every candidate cell has rho > 0 and p > 0 (p from E, m and the
+// cell-centred B, gamma = 5/3).
+__global__ __launch_bounds__(256) void check_kernel(const float *__restrict__ u,
+                                                    const float *__restrict__ uc,
+                                                    const float *__restrict__ bc, int nx, int ny,
+                                                    int nz, double *__restrict__ part)
+{
+    __shared__ double e0[256], e1[256];
+    __shared__ int okb[256];
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, c = 0.0;
+    int ok = 1;
+    if (v < (long)nx * ny * nz) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
+        a = u[v * 5 + 4];
+        const float *q = uc + v * 5;
+        c = q[4];
+        const float b1 = 0.5f * (bc[v * 3] + bc[vid(wrap(i + 1, nx), j, k, nx, ny) * 3]);
+        const float b2 = 0.5f * (bc[v * 3 + 1] + bc[vid(i, wrap(j + 1, ny), k, nx, ny) * 3 + 1]);
+        const float b3 = 0.5f * (bc[v * 3 + 2] + bc[vid(i, j, wrap(k + 1, nz), nx, ny) * 3 + 2]);
+        const double rho = q[0];
+        const double kin = 0.5 * ((double)q[1] * q[1] + (double)q[2] * q[2] + (double)q[3] * q[3]) / rho;
+        const double p = (2.0 / 3.0) * ((double)q[4] - kin - 0.5 * ((double)b1 * b1 + (double)b2 * b2 + (double)b3 * b3));
+        ok = rho > 0.0 && p > 0.0;  // false for NaN too
+    }
+    e0[threadIdx.x] = a;
+    e1[threadIdx.x] = c;
+    okb[threadIdx.x] = ok;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            e0[threadIdx.x] += e0[threadIdx.x + w];
+            e1[threadIdx.x] += e1[threadIdx.x + w];
+            okb[threadIdx.x] &= okb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = e0[0];
+        part[3 * blockIdx.x + 1] = e1[0];
+        part[3 * blockIdx.x + 2] = okb[0];
+    }
+}
+
+// One workgroup: fold the block partials in order, decide, count.
+// stats[0] += accepted, stats[1] += rejected; flag[0] = accepted.
+__global__ __launch_bounds__(256) void decide_kernel(const double *__restrict__ part, int nblk,
+                                                     double etol, int *__restrict__ flag,
+                                                     int *__restrict__ stats)
+{
+    __shared__ double e0[256], e1[256];
+    __shared__ int okb[256];
+    double a = 0.0, c = 0.0;
+    int ok = 1;
+    for (int q = threadIdx.x; q < nblk; q += 256) {
+        a += part[3 * q];
+        c += part[3 * q + 1];
+        ok &= part[3 * q + 2] != 0.0;
+    }
+    e0[threadIdx.x] = a;
+    e1[threadIdx.x] = c;
+    okb[threadIdx.x] = ok;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            e0[threadIdx.x] += e0[threadIdx.x + w];
+            e1[threadIdx.x] += e1[threadIdx.x + w];
+            okb[threadIdx.x] &= okb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const bool acc = okb[0] && fabs(e1[0] - e0[0]) <= etol * fabs(e0[0]);
+        flag[0] = acc;
+        stats[acc ? 0 : 1] += 1;
+    }
+}
+
+__global__ void commit_kernel(float *__restrict__ u, float *__restrict__ b, const float *__restrict__ uc,
+                              const float *__restrict__ bc, long n, const int *__restrict__ flag)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n || !flag[0]) return;
+#pragma unroll
+    for (int c = 0; c < 5; c++) u[v * 5 + c] = uc[v * 5 + c];
+#pragma unroll
+    for (int d = 0; d < 3; d++) b[v * 3 + d] = bc[v * 3 + d];
+}
+
+int done() { return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP; }
+unsigned blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+bool conv_shape_ok(int nx, int ny, int nz)
+{
+    const long n = (long)nx * ny * nz;
+    return nx >= 64 && nx % 64 == 0 && ny >= 1 && nz >= 1 && n % 256 == 0 && n / 256 < 0x7fffffffL;
+}
+
+}  // namespace
+}  // namespace mhd
+
+using namespace mhd;
+
+extern "C" {
+
+int mhd_corr_num_params(void) { return kParams; }
+
+int mhd_corr_conv(const uint16_t *x, const uint16_t *w, const float *bias, int nx, int ny, int nz,
+                  int mode, float *z_out, uint16_t *a_out, const float *z_prev, mhd_stream_t stream)
+{
+    if (!x || !w || !conv_shape_ok(nx, ny, nz)) return MHD_ERR_ARG;
+    const hipStream_t s = (hipStream_t)stream;
+    const unsigned grid = (unsigned)((long)nx * ny * nz / 256);
+    (void)hipGetLastError();
+    switch (mode) {
+    case kModeHidden:
+        if (!bias || !z_out || !a_out) return MHD_ERR_ARG;
+        hipLaunchKernelGGL(conv_kernel<kModeHidden>, dim3(grid), dim3(256), 0, s, x, w, bias, nx, ny,
+                           nz, z_out, a_out, nullptr);
+        break;
+    case kModeOutput:
+        if (!bias || !z_out) return MHD_ERR_ARG;
+        hipLaunchKernelGGL(conv_kernel<kModeOutput>, dim3(grid), dim3(256), 0, s, x, w, bias, nx, ny,
+                           nz, z_out, nullptr, nullptr);
+        break;
+    case kModeGrad:
+        if (!a_out || !z_prev) return MHD_ERR_ARG;
+        hipLaunchKernelGGL(conv_kernel<kModeGrad>, dim3(grid), dim3(256), 0, s, x, w, nullptr, nx, ny,
+                           nz, nullptr, a_out, z_prev);
+        break;
+    default: return MHD_ERR_ARG;
+    }
+    return done();
+}
+
+size_t mhd_corr_dw_workspace_bytes(void) { return sizeof(float) * (size_t)kDwGroups * kParams; }
+
+int mhd_corr_conv_dw(const uint16_t *dy, const uint16_t *x, int nx, int ny, int nz, float *ws,
+                     int accumulate, mhd_stream_t stream)
+{
+    if (!dy || !x || !ws || !conv_shape_ok(nx, ny, nz)) return MHD_ERR_ARG;
+    const size_t lds = sizeof(u16) * ((size_t)kC * nx + 9 * (size_t)kC * (nx + 2));
+    if (lds > 160 * 1024) return MHD_ERR_TOO_LONG;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)conv_dw_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(conv_dw_kernel, dim3(kDwGroups), dim3(256), lds, (hipStream_t)stream, dy, x,
+                       nx, ny, nz, ws, accumulate);
+    return done();
+}
+
+int mhd_corr_dw_reduce(const float *ws, float *grad, mhd_stream_t stream)
+{
+    if (!ws || !grad) return MHD_ERR_ARG;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(dw_reduce_kernel, dim3(blocks(kParams, 256)), dim3(256), 0, (hipStream_t)stream,
+                       ws, kDwGroups, grad);
+    return done();
+}
+
+int mhd_corr_weight_prep(const float *w, uint16_t *wf, uint16_t *wb, mhd_stream_t stream)
+{
+    if (!w || !wf || !wb) return MHD_ERR_ARG;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(weight_prep_kernel, dim3(blocks(kWElems, 256)), dim3(256), 0,
+                       (hipStream_t)stream, w, wf, wb);
+    return done();
+}
+
+int mhd_corr_adam(float *p, float *g, float *m, float *v, int n, float lr, float beta1, float beta2,
+                  float eps, int step, float gscale, mhd_stream_t stream)
+{
+    if (!p || !g || !m || !v || n < 1 || step < 1) return MHD_ERR_ARG;
+    const float c1 = (float)(1.0 / (1.0 - pow((double)beta1, step)));
+    const float c2 = (float)(1.0 / (1.0 - pow((double)beta2, step)));
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, p, g, m,
+                       v, n, lr, beta1, beta2, eps, c1, c2, gscale);
+    return done();
+}
+
+int mhd_corr_pack(const float *u, const float *b, int nx, int ny, int nz, const float *norm,
+                  uint16_t *x, mhd_stream_t stream)
+{
+    if (!u || !b || !norm || !x || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks((long)nx * ny * nz, 256)), dim3(256), 0,
+                       (hipStream_t)stream, u, b, nx, ny, nz, norm, x);
+    return done();
+}
+
+size_t mhd_corr_loss_partials(int nx, int ny, int nz) { return blocks((long)nx * ny * nz, 256); }
+
+int mhd_corr_loss_grad(const float *y, const float *target, int nx, int ny, int nz, float *r_scratch,
+                       double *loss_partials, uint16_t *dy, mhd_stream_t stream)
+{
+    if (!y || !target || !r_scratch || !loss_partials || !dy || nx < 1 || ny < 1 || nz < 1)
+        return MHD_ERR_ARG;
+    const long n = (long)nx * ny * nz;
+    const hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(residual_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, y, target, nx, ny, nz,
+                       r_scratch, loss_partials);
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, r_scratch, nx, ny, nz,
+                       (float)(2.0 / (double)n), dy);
+    return done();
+}
+
+int mhd_downsample(const float *u, const float *b, int nx, int ny, int nz, int r, float *uc,
+                   float *bc, mhd_stream_t stream)
+{
+    if (!u || !b || !uc || !bc || r < 1 || nx % r || ny % r || nz % r || nx < 1 || ny < 1 || nz < 1)
+        return MHD_ERR_ARG;
+    const long n = (long)(nx / r) * (ny / r) * (nz / r);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(downsample_kernel, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, u,
+                       b, nx, ny, nz, r, uc, bc);
+    return done();
+}
+
+size_t mhd_corr_check_bytes(int nx, int ny, int nz)
+{
+    return 5 * sizeof(double) * blocks((long)nx * ny * nz, 256) + 64;
+}
+
+int mhd_corr_apply(float *u, float *b, const float *y, int nx, int ny, int nz, const float *scale,
+                   double etol, float *u_cand, float *b_cand, void *check_ws, int *flag_stats,
+                   mhd_stream_t stream)
+{
+    if (!u || !b || !y || !scale || !u_cand || !b_cand || !check_ws || !flag_stats || nx < 1 ||
+        ny < 1 || nz < 1)
+        return MHD_ERR_ARG;
+    const long n = (long)nx * ny * nz;
+    const unsigned nb = blocks(n, 256);
+    const hipStream_t s = (hipStream_t)stream;
+    double *part = (double *)check_ws;  // 5 doubles per block, reused by the check
+    float *ymean = (float *)(part + 5 * (size_t)nb);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(ysum_kernel, dim3(nb), dim3(256), 0, s, y, n, part);
+    hipLaunchKernelGGL(ymean_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, n, ymean);
+    hipLaunchKernelGGL(apply_kernel, dim3(nb), dim3(256), 0, s, u, b, y, nx, ny, nz, scale, ymean,
+                       u_cand, b_cand);
+    hipLaunchKernelGGL(check_kernel, dim3(nb), dim3(256), 0, s, u, u_cand, b_cand, nx, ny, nz, part);
+    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, etol, flag_stats,
+                       flag_stats + 1);
+    hipLaunchKernelGGL(commit_kernel, dim3(nb), dim3(256), 0, s, u, b, u_cand, b_cand, n, flag_stats);
+    return done();
+}
+
+}  // extern "C"

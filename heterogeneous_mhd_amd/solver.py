@@ -131,6 +131,33 @@ class Box:
         self.stream.synchronize()
         check(lib().mhd_plan_set_time(self._plan, 0.0), "set_time")
 
+    def set_time(self, t: float):
+        """Sets t; the next run recomputes dt from the state."""
+        self.stream.synchronize()
+        check(lib().mhd_plan_set_time(self._plan, float(t)), "set_time")
+
+    def set_tf(self, tf: float):
+        """End time of the clamp (0 disables it)."""
+        check(lib().mhd_plan_set_tf(self._plan, float(tf)), "plan_set_tf")
+
+    def state_changed(self):
+        """Call after modifying u/b outside run(): the next step recomputes dt."""
+        check(lib().mhd_plan_state_changed(self._plan), "plan_state_changed")
+
+    def advance_to(self, t_end: float, max_steps: int = 1 << 20) -> int:
+        """Steps until t reaches t_end exactly (the clamp makes the last step
+        land on it); returns the number of steps taken."""
+        self.set_tf(t_end)
+        steps = 0
+        while steps < max_steps:
+            t, dt = self.time()
+            if t >= t_end or (steps and dt == 0.0):
+                break
+            self.run(1, graph=False)
+            steps += 1
+        self.set_tf(0.0)
+        return steps
+
     # -- stepping ----------------------------------------------------------
     def run(self, nsteps: int, graph: bool = False):
         check(lib().mhd_plan_run(self._plan, self.u.data_ptr(), self.b.data_ptr(),

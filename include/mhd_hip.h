@@ -145,6 +145,12 @@ int mhd_plan_set_time(mhd_plan *plan, float t);
 int mhd_plan_run(mhd_plan *plan, float *u, float *b, float *u2, float *b2, int nsteps, int use_graph);
 /* Selects MHD_PREC_EXACT or MHD_PREC_FAST for the plan's later steps. */
 int mhd_plan_set_precision(mhd_plan *plan, int mode);
+/* Changes the end time of the clamp (tf > 0: dt <= (tf - t)/2, so t lands on
+ * tf and later steps have dt = 0). */
+int mhd_plan_set_tf(mhd_plan *plan, float tf);
+/* Tells the plan that (u, b) was modified outside its steps: the next run
+ * recomputes calcfl before its first step. */
+int mhd_plan_state_changed(mhd_plan *plan);
 /* Copies t and dt to the host (synchronises the plan's stream). */
 int mhd_plan_get_time(mhd_plan *plan, float *t, float *dt);
 /* Synchronises the plan's stream. */
@@ -211,6 +217,66 @@ int mhd_ensemble_get_config(mhd_ensemble *ens, int *shape, int *grid, int *block
 int mhd_ensemble_tune(mhd_ensemble *ens, int reps, float *best_ms);
 /* Selects MHD_PREC_EXACT or MHD_PREC_FAST for the ensemble's later steps. */
 int mhd_ensemble_set_precision(mhd_ensemble *ens, int mode);
+
+/* ------------------------------------------------------------------ */
+/* Part 3: learned coarse-grid corrector (no counterpart in the reference) */
+/* ------------------------------------------------------------------ */
+/* A 4-layer periodic 3x3x3 convolution network, 32 channels, GELU, whose
+ * output corrects a coarse step: u += scale[c] * y[c] (c < 5) and
+ * b += scale[5] * curl(y[5..7]) (a vector-potential correction, so div b
+ * stays at round-off).  Activations are bf16 [V][32] (channel fastest, V =
+ * nx*ny*nz in state order, physical channels rho, m1..m3, E, cell-centred
+ * B1..B3, zero-padded); weights bf16 [27][32 out][32 in] with tap
+ * (dz+1)*9 + (dy+1)*3 + (dx+1).  Products on BF16 MFMA, FP32 accumulate.
+ * Convolutions need nx a multiple of 64 and nx*ny*nz a multiple of 256. */
+
+/* floats per layer: 27*32*32 weights followed by 32 biases */
+int mhd_corr_num_params(void);
+/* mode 0: z_out = conv(x) + bias (FP32 [V][32]), a_out = bf16(GELU(z));
+ * mode 1: z_out = conv(x) + bias;
+ * mode 2: a_out = bf16(conv(x) * GELU'(z_prev)) -- with the input-gradient
+ *         weights of mhd_corr_weight_prep, the backward pass of a layer. */
+int mhd_corr_conv(const uint16_t *x, const uint16_t *w, const float *bias, int nx, int ny, int nz,
+                  int mode, float *z_out, uint16_t *a_out, const float *z_prev, mhd_stream_t stream);
+/* Weight gradient of one layer: partial sums of dW[t][o][c] = sum_v dy[v][o] *
+ * x[v + off_t][c] and of the bias gradient, per workgroup, into ws
+ * (mhd_corr_dw_workspace_bytes()); accumulate != 0 adds to what ws holds.
+ * mhd_corr_dw_reduce adds the folded sums to grad (num_params floats).
+ * Deterministic. */
+size_t mhd_corr_dw_workspace_bytes(void);
+int mhd_corr_conv_dw(const uint16_t *dy, const uint16_t *x, int nx, int ny, int nz, float *ws,
+                     int accumulate, mhd_stream_t stream);
+int mhd_corr_dw_reduce(const float *ws, float *grad, mhd_stream_t stream);
+/* FP32 master weights -> bf16 forward form wf and input-gradient form wb. */
+int mhd_corr_weight_prep(const float *w, uint16_t *wf, uint16_t *wb, mhd_stream_t stream);
+/* Adam step on n parameters (gradient scaled by gscale, then zeroed). */
+int mhd_corr_adam(float *p, float *g, float *m, float *v, int n, float lr, float beta1, float beta2,
+                  float eps, int step, float gscale, mhd_stream_t stream);
+/* State -> network input; norm[0..7] means, norm[8..15] inverse std devs. */
+int mhd_corr_pack(const float *u, const float *b, int nx, int ny, int nz, const float *norm,
+                  uint16_t *x, mhd_stream_t stream);
+/* Loss = mean over voxels of |y_u - t_u|^2 + |curl(y_A) - t_B|^2 with
+ * target [V][8] (normalised u correction, normalised b correction); writes
+ * per-block loss sums (mhd_corr_loss_partials() doubles), the residual
+ * (r_scratch, [V][8] floats) and dy = dLoss/dy (bf16 [V][32]). */
+size_t mhd_corr_loss_partials(int nx, int ny, int nz);
+int mhd_corr_loss_grad(const float *y, const float *target, int nx, int ny, int nz, float *r_scratch,
+                       double *loss_partials, uint16_t *dy, mhd_stream_t stream);
+/* Block average by r of a fine state: u over r^3 cells, each face component
+ * over the r^2 fine faces of the coarse face (div b stays 0). */
+int mhd_downsample(const float *u, const float *b, int nx, int ny, int nz, int r, float *uc,
+                   float *bc, mhd_stream_t stream);
+/* Applies the correction y (FP32 [V][32] output layer) to (u, b) -- the u
+ * part projected to zero box total, as the scheme conserves the totals of
+ * rho, m and E -- unless the corrected state fails a check: rho > 0 and p > 0 in every cell and
+ * |sum E - sum E_before| <= etol * |sum E_before| (FP64 sums).  u_cand,
+ * b_cand: scratch of the state's sizes; check_ws: mhd_corr_check_bytes();
+ * flag_stats: 3 device ints: [0] = this call accepted, [1] += accepted,
+ * [2] += rejected.  Entirely on the device (graph-capturable). */
+size_t mhd_corr_check_bytes(int nx, int ny, int nz);
+int mhd_corr_apply(float *u, float *b, const float *y, int nx, int ny, int nz, const float *scale,
+                   double etol, float *u_cand, float *b_cand, void *check_ws, int *flag_stats,
+                   mhd_stream_t stream);
 
 #ifdef __cplusplus
 }

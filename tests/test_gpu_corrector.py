@@ -1,0 +1,225 @@
+"""The corrector's BF16 MFMA kernels and host passes against the CPU oracle
+(oracle/corrector_ref.py, pinned in test_corrector_oracle.py).
+
+Tolerances: FP32 outputs (pre-activations, weight gradients) within 1e-5 of
+the largest magnitude (the oracle sums in FP64, the MFMA in FP32); bf16
+outputs within one bf16 ulp (at most 2^-7 relative) of the oracle's,
+because an FP32 value a few ulp from the oracle's can round to the
+neighbouring bf16 value."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import corrector_ref as R
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(77)
+
+
+def _bits(x):
+    """float array -> bf16 bit pattern as int16 (nearest even)."""
+    a = np.ascontiguousarray(R.bf16(x).astype(np.float32))
+    return (a.view(np.uint32) >> 16).astype(np.uint16).view(np.int16)
+
+
+def _from_bits(h):
+    return (np.asarray(h).view(np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _dev(a, dtype=None):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _grid(flat, nx, ny, nz, c):
+    return np.asarray(flat).reshape(nz, ny, nx, c)
+
+
+SHAPES = [(64, 4, 4), (128, 2, 2), (64, 3, 4)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_conv_forward_and_input_gradient(gpu, shape):
+    import torch
+    from heterogeneous_mhd_amd._lib import check, lib
+    nx, ny, nz = shape
+    print(f"conv kernels {shape}", flush=True)
+    v = nx * ny * nz
+    x = R.bf16(RNG.standard_normal((nz, ny, nx, 32)))
+    w = RNG.standard_normal((27, 32, 32)) * 0.05
+    bias = RNG.standard_normal(32) * 0.1
+    params = np.concatenate([w.reshape(-1), bias]).astype(np.float32)
+    p = _dev(params)
+    wf = torch.empty(27 * 1024, dtype=torch.int16, device="cuda")
+    wb = torch.empty_like(wf)
+    L = lib()
+    s = torch.cuda.current_stream().cuda_stream
+    check(L.mhd_corr_weight_prep(p.data_ptr(), wf.data_ptr(), wb.data_ptr(), s), "prep")
+    wq = R.bf16(params[:27 * 1024]).reshape(27, 32, 32)
+    assert np.array_equal(_from_bits(wf.cpu().numpy()).reshape(27, 32, 32), wq)
+    xd = _dev(_bits(x).reshape(-1))
+    z = torch.empty(v * 32, dtype=torch.float32, device="cuda")
+    a = torch.empty(v * 32, dtype=torch.int16, device="cuda")
+    check(L.mhd_corr_conv(xd.data_ptr(), wf.data_ptr(), p[27 * 1024:].data_ptr(), nx, ny, nz, 0,
+                          z.data_ptr(), a.data_ptr(), None, s), "conv")
+    zr = R.conv(x, wq, params[27 * 1024:].astype(np.float64))
+    zg = _grid(z.cpu().numpy(), nx, ny, nz, 32)
+    assert np.abs(zg - zr).max() <= 1e-5 * np.abs(zr).max()
+    ag = _from_bits(_grid(a.cpu().numpy(), nx, ny, nz, 32))
+    ar = R.bf16(R.gelu(zg))
+    assert np.all(np.abs(ag - ar) <= 2.0 ** -7 * np.abs(ar) + 1e-7)
+    # output mode
+    check(L.mhd_corr_conv(xd.data_ptr(), wf.data_ptr(), p[27 * 1024:].data_ptr(), nx, ny, nz, 1,
+                          z.data_ptr(), None, None, s), "conv out")
+    assert np.abs(_grid(z.cpu().numpy(), nx, ny, nz, 32) - zr).max() <= 1e-5 * np.abs(zr).max()
+    # input gradient through GELU: d_in = conv^T(d_out, w) * GELU'(z_prev)
+    d = R.bf16(RNG.standard_normal((nz, ny, nx, 32)))
+    zp = RNG.standard_normal((nz, ny, nx, 32)).astype(np.float32)
+    dd, zpd = _dev(_bits(d).reshape(-1)), _dev(zp.reshape(-1))
+    check(L.mhd_corr_conv(dd.data_ptr(), wb.data_ptr(), None, nx, ny, nz, 2, None, a.data_ptr(),
+                          zpd.data_ptr(), s), "conv grad")
+    gr = R.conv_input_grad(d, wq) * R.gelu_grad(zp.astype(np.float64))
+    gg = _from_bits(_grid(a.cpu().numpy(), nx, ny, nz, 32))
+    assert np.all(np.abs(gg - R.bf16(gr)) <= 2.0 ** -7 * np.abs(gr) + 1e-6 * np.abs(gr).max())
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_weight_gradient_and_accumulation(gpu, shape):
+    import torch
+    from heterogeneous_mhd_amd._lib import check, lib
+    nx, ny, nz = shape
+    print(f"weight gradient {shape}", flush=True)
+    L = lib()
+    s = torch.cuda.current_stream().cuda_stream
+    dy = R.bf16(RNG.standard_normal((nz, ny, nx, 32)))
+    x = R.bf16(RNG.standard_normal((nz, ny, nx, 32)))
+    ws = torch.empty(L.mhd_corr_dw_workspace_bytes() // 4, dtype=torch.float32, device="cuda")
+    g = torch.zeros(L.mhd_corr_num_params(), dtype=torch.float32, device="cuda")
+    dyd, xd = _dev(_bits(dy).reshape(-1)), _dev(_bits(x).reshape(-1))
+    check(L.mhd_corr_conv_dw(dyd.data_ptr(), xd.data_ptr(), nx, ny, nz, ws.data_ptr(), 0, s), "dw")
+    check(L.mhd_corr_conv_dw(dyd.data_ptr(), xd.data_ptr(), nx, ny, nz, ws.data_ptr(), 1, s), "dw acc")
+    check(L.mhd_corr_dw_reduce(ws.data_ptr(), g.data_ptr(), s), "reduce")
+    dw, db = R.conv_weight_grad(dy, x)
+    gh = g.cpu().numpy().astype(np.float64)
+    assert np.abs(gh[:27 * 1024].reshape(27, 32, 32) - 2 * dw).max() <= 1e-5 * np.abs(2 * dw).max()
+    assert np.abs(gh[27 * 1024:] - 2 * db).max() <= 1e-5 * np.abs(2 * db).max()
+
+
+def test_network_gradients_match_oracle(gpu):
+    """One forward + backward of the whole network (pack, 4 layers, curl loss)
+    on the GPU against the oracle with the same bf16 roundings."""
+    import torch
+    from heterogeneous_mhd_amd import corrector
+    nx, ny, nz = 64, 8, 4
+    print("network gradient 64x8x4", flush=True)
+    net = corrector.CorrectorNet(seed=5)
+    # a state near rest with fluctuations, a random target
+    u = np.zeros((nz, ny, nx, 5))
+    u[..., 0] = 1.0 + 0.1 * RNG.standard_normal((nz, ny, nx))
+    u[..., 1:4] = 0.1 * RNG.standard_normal((nz, ny, nx, 3))
+    u[..., 4] = 1.0 + 0.1 * RNG.standard_normal((nz, ny, nx))
+    b = 0.05 * RNG.standard_normal((nz, ny, nx, 3))
+    tgt = RNG.standard_normal((nz, ny, nx, 8))
+    norm = np.concatenate([[1.0, 0, 0, 0, 1.0, 0, 0, 0], [10.0] * 5 + [20.0] * 3]).astype(np.float32)
+    net.norm.copy_(torch.from_numpy(norm))
+    buf = corrector.Buffers(nx, ny, nz, net.device, train=True)
+    ud, bd = _dev(u.astype(np.float32).reshape(-1)), _dev(b.astype(np.float32).reshape(-1))
+    td = _dev(tgt.astype(np.float32).reshape(-1, 8))
+    y = net.forward(buf, ud, bd)
+    loss = float(net.backward(buf, td, accumulate=False))
+    grads = net.gradients(buf).cpu().numpy().astype(np.float64)
+
+    P = net.params.cpu().numpy().astype(np.float64)
+    onet = R.Net([(P[li, :27 * 1024].reshape(27, 32, 32), P[li, 27 * 1024:]) for li in range(4)])
+    x0 = R.pack(u.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64),
+                norm[:8].astype(np.float64), norm[8:].astype(np.float64))
+    yr, cache = onet.forward(x0)
+    yg = y.cpu().numpy().reshape(nz, ny, nx, 32)
+    assert np.abs(yg - yr).max() <= 1e-3 * np.abs(yr).max()
+    lr_, g = R.loss_and_grad(yr, tgt.astype(np.float32).astype(np.float64))
+    assert abs(loss - lr_) <= 1e-4 * abs(lr_)
+    og = onet.backward(g, cache)
+    for li in range(4):
+        gw = grads[li, :27 * 1024].reshape(27, 32, 32)
+        err = np.linalg.norm(gw - og[li][0]) / np.linalg.norm(og[li][0])
+        errb = np.linalg.norm(grads[li, 27 * 1024:] - og[li][1]) / np.linalg.norm(og[li][1])
+        print(f"layer {li}: weight-gradient rel err {err:.2e}, bias {errb:.2e}", flush=True)
+        assert err <= 5e-3 and errb <= 5e-3
+
+
+def test_downsample_matches_oracle_and_keeps_divb(gpu):
+    import torch
+    from heterogeneous_mhd_amd import corrector, solver
+    n = 64
+    print("downsample 64 -> 16 and 32", flush=True)
+    u, b = solver.ic_device(n, n, n, solver.ICSpec(seed=8, amp_v=0.3, amp_a=0.1))
+    uh, bh = R.grid(u.cpu().numpy(), b.cpu().numpy(), n, n, n)
+    for r in (2, 4):
+        uc, bc = corrector.downsample(u, b, n, r)
+        torch.cuda.synchronize()
+        ur, br = R.downsample(uh, bh, r)
+        m = n // r
+        ug, bg = R.grid(uc.cpu().numpy(), bc.cpu().numpy(), m, m, m)
+        assert np.all(np.abs(ug - ur) <= 2.0 ** -23 * np.abs(ur))
+        assert np.all(np.abs(bg - br) <= 2.0 ** -23 * np.abs(br) + 1e-12)
+        assert np.abs(R.div_faces(bg)).max() <= 1e-6
+
+
+def test_apply_accepts_and_falls_back(gpu):
+    import torch
+    from heterogeneous_mhd_amd import corrector
+    nx, ny, nz = 64, 4, 4
+    print("apply / fallback", flush=True)
+    net = corrector.CorrectorNet()
+    net.scale.copy_(torch.tensor([0.5, 0.5, 0.5, 0.5, 0.5, 0.2]))
+    buf = corrector.Buffers(nx, ny, nz, net.device, train=False)
+    u = np.zeros((nz, ny, nx, 5), np.float32)
+    u[..., 0], u[..., 4] = 1.0, 2.0
+    b = (0.01 * RNG.standard_normal((nz, ny, nx, 3))).astype(np.float32)
+    y = (0.01 * RNG.standard_normal((nz, ny, nx, 32))).astype(np.float32)
+    stats = torch.zeros(3, dtype=torch.int32, device="cuda")
+    scale = np.array([0.5] * 5 + [0.2])
+    for case, etol in (("accept", 1e-2), ("negative p", 1e-2), ("negative rho", 1e-2)):
+        yy = y.copy()
+        if case == "negative rho":
+            yy[1, 2, 3, 0] = -10.0
+        if case == "negative p":
+            yy[2, 1, 7, 4] = -10.0
+        ud, bd, yd = _dev(u.reshape(-1)), _dev(b.reshape(-1)), _dev(yy.reshape(-1))
+        net.apply(buf, ud, bd, yd, etol, stats)
+        ur, br, ok = R.apply_correction(u.astype(np.float64), b.astype(np.float64),
+                                        yy.astype(np.float64), scale, etol)
+        assert ok == (case == "accept"), case
+        assert int(stats[0]) == int(ok)
+        ug = ud.cpu().numpy().reshape(nz, ny, nx, 5)
+        bg = bd.cpu().numpy().reshape(nz, ny, nx, 3)
+        assert np.allclose(ug, ur, rtol=1e-6, atol=1e-7) and np.allclose(bg, br, rtol=1e-6, atol=1e-7)
+    assert stats.cpu().tolist()[1:] == [1, 2]
+
+
+def test_advance_to_lands_on_the_time(gpu):
+    from heterogeneous_mhd_amd import solver
+    box = solver.Box(32, 32, 32, cfl=0.7)
+    box.init(solver.ICSpec(seed=3, amp_v=0.2))
+    n = box.advance_to(3.3)
+    t, _ = box.time()
+    box.close()
+    assert n > 0 and np.float32(t) == np.float32(3.3)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..",
+                                                    "heterogeneous_mhd_amd", "data",
+                                                    "corrector_64.npz")),
+                    reason="no trained corrector weights in the tree")
+def test_trained_corrector_beats_plain_coarse_step(gpu):
+    """On a held-out seed, the corrected 64^3 run is closer to the 4x
+    downsampled 256^3 run than the plain 64^3 run after 40 coarse steps."""
+    from heterogeneous_mhd_amd import corrector
+    print("trained corrector, held-out seed 5000, 40 steps", flush=True)
+    net = corrector.CorrectorNet.load()
+    rows = corrector.evaluate(net, [5000], 40, log=print)
+    r = rows[0]
+    assert r["corrected"]["mean"] < r["plain"]["mean"]
+    assert r["fallbacks"]["rejected"] <= 40
