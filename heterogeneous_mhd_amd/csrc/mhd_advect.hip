@@ -4,7 +4,7 @@
 namespace mhd {
 namespace {
 
-template <int BLOCK, int M, bool FAST>
+template <int BLOCK, int M, bool FAST, int TP, bool FIT>
 __global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out, const float *b_main,
                                                        const float *__restrict__ uu,
                                                        const float *__restrict__ bu, int nx,
@@ -14,23 +14,25 @@ __global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out, const float
     extern __shared__ float lds[];
     const int pb = BLOCK / tp;
     const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
-    advect_group<BLOCK, M, FAST>(lds, b_out, b_main, uu, bu, nx, ny, nz, tp, grp, dtp ? *dtp : dtv);
+    float keep[M][5];
+    advect_group<BLOCK, M, FAST, TP, FIT>(lds, b_out, b_main, uu, bu, nx, ny, nz, tp, grp,
+                                          dtp ? *dtp : dtv, keep);
 }
 
-template <int BLOCK, int M, bool FAST>
+template <int BLOCK, int M, int TP, bool FAST>
 int launch_cfg(float *b_out, const float *b_main, const float *uu, const float *bu, int nx, int ny,
-               int nz, int tp, const float *dtp, float dtv, hipStream_t s)
+               int nz, const float *dtp, float dtv, hipStream_t s)
 {
-    const int pb = BLOCK / tp;
+    constexpr int pb = BLOCK / TP;
     const long npen = (long)ny * nz;
     const long nblk = (npen + pb - 1) / pb;
     const size_t lds = sizeof(float) * 3 * kLines * (size_t)nx * pb;
+    auto k = nx == M * TP ? advect_kernel<BLOCK, M, FAST, TP, true> : advect_kernel<BLOCK, M, FAST, TP, false>;
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)advect_kernel<BLOCK, M, FAST>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
-    hipLaunchKernelGGL((advect_kernel<BLOCK, M, FAST>), dim3((unsigned)nblk), dim3(BLOCK), lds, s, b_out,
-                       b_main, uu, bu, nx, ny, nz, tp, dtp, dtv);
+    hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(BLOCK), lds, s, b_out, b_main, uu, bu, nx, ny, nz,
+                       TP, dtp, dtv);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -38,12 +40,12 @@ template <bool FAST>
 int launch_any(float *b_out, const float *b_main, const float *uu, const float *bu, int nx, int ny,
                int nz, const float *dt_dev, float dt_val, hipStream_t s)
 {
-    if (nx <= 64) return launch_cfg<256, 1, FAST>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
-    if (nx <= 128) return launch_cfg<256, 2, FAST>(b_out, b_main, uu, bu, nx, ny, nz, 64, dt_dev, dt_val, s);
-    if (nx <= 256) return launch_cfg<256, 2, FAST>(b_out, b_main, uu, bu, nx, ny, nz, 128, dt_dev, dt_val, s);
-    if (nx <= 512) return launch_cfg<256, 2, FAST>(b_out, b_main, uu, bu, nx, ny, nz, 256, dt_dev, dt_val, s);
-    if (nx <= 1024) return launch_cfg<512, 2, FAST>(b_out, b_main, uu, bu, nx, ny, nz, 512, dt_dev, dt_val, s);
-    return launch_cfg<1024, 2, FAST>(b_out, b_main, uu, bu, nx, ny, nz, 1024, dt_dev, dt_val, s);
+    if (nx <= 64) return launch_cfg<256, 1, 64, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    if (nx <= 128) return launch_cfg<256, 2, 64, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    if (nx <= 256) return launch_cfg<256, 2, 128, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    if (nx <= 512) return launch_cfg<256, 2, 256, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    if (nx <= 1024) return launch_cfg<512, 2, 512, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    return launch_cfg<1024, 2, 1024, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
 }
 
 }  // namespace

@@ -27,14 +27,28 @@ namespace {
 
 constexpr int kLines = 4;  // A(j), A(j+1), B(k), B(k+1)
 
-template <int BLOCK, int M, bool FAST>
+// keep[r] receives, for the thread's r-th cell, the new (b1, b2, b3) of this
+// pencil and the new b2 of pencil (j+1, k) and b3 of pencil (j, k+1) -- the
+// lines A(j+1) and B(k+1) this workgroup evaluates are the ones those
+// pencils' own b2 and b3 come from, computed in the same order -- so a fused
+// fluid sweep can follow without reading b back (see mhd_fused.hip).
+//
+// TPC > 0 fixes the threads per pencil at compile time; FIT (with TPC) also
+// fixes nx = M*TPC, so that LDS offsets are immediates and there is no
+// bounds test.  The pencil index is uniform over a wavefront (tp is a
+// multiple of 64), so it is taken from the first lane and the per-pencil
+// address arithmetic runs on the scalar unit.
+template <int BLOCK, int M, bool FAST, int TPC = 0, bool FIT = false>
 __device__ __forceinline__ void advect_group(float *lds, float *b_out, const float *b_main,
                                              const float *__restrict__ uu,
-                                             const float *__restrict__ bu, int nx, int ny, int nz,
-                                             int tp, int grp, float dt)
+                                             const float *__restrict__ bu, int nx_, int ny, int nz,
+                                             int tp_, int grp, float dt, float (&keep)[M][5])
 {
+    static_assert(!FIT || TPC > 0, "FIT needs the threads per pencil at compile time");
+    const int tp = TPC > 0 ? TPC : tp_;
+    const int nx = FIT ? M * TPC : nx_;
     const int pb = BLOCK / tp;
-    const int lp = threadIdx.x / tp;
+    const int lp = __builtin_amdgcn_readfirstlane(threadIdx.x / tp);
     const int lt = threadIdx.x - lp * tp;
     const int npen = ny * nz;
     const int pen = grp * pb + lp;
@@ -215,6 +229,11 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
             o[0] = b1n;
             o[1] = b2n;
             o[2] = b3n;
+            keep[r][0] = b1n;
+            keep[r][1] = b2n;
+            keep[r][2] = b3n;
+            keep[r][3] = bl[r][1] - (fx[r][1] - fm[1]);  // b2 of pencil (j+1, k)
+            keep[r][4] = bl[r][3] - (fx[r][3] - fm[3]);  // b3 of pencil (j, k+1)
         }
     }
 }

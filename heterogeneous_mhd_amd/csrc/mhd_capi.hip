@@ -422,10 +422,11 @@ int mhd_plan_set_precision(mhd_plan *p, int mode)
 }
 
 // One step, C/mhd.c:59-124.  Operators map state -> state; the reference's
-// copy_matrix calls disappear into a ping-pong between (U0,B0) and (U1,B1).
-// fluidx works in place; advectbyzx and the transposes write the other
-// buffer.  u changes buffer 4 times and b 10 times per step, so the state
-// ends where it started.  calcfl (:62) of the next step is fused into this
+// copy_matrix calls disappear into a ping-pong between (U0,B0) and (U1,B1),
+// and the three advectbyzx + fluidx pairs of the second half run fused.
+// The y and z fluid sweeps of the first half work in place; every other
+// operator writes the other buffer.  u changes buffer 8 times and b 10 times
+// per step, so the state ends where it started.  calcfl (:62) of the next step is fused into this
 // step's last fluid sweep, which writes 1/c and the unclamped dt to scal[2..3];
 // the step begins by applying the clamp and advancing t (:63-64).
 static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
@@ -440,27 +441,28 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
         if (rc == MHD_OK) rc = (x); \
     } while (0)
     RUN(mhd::launch_begin_step(t, dt, dtn, p->tf, s));                                  // :63-64
-    // x sweep
-    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :71
-    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, nx, ny, nz, dt, 0.0f, s, f));               // :74
-    RUN(mhd::launch_transpose12(U1, B0, U0, B1, nx, ny, nz, s));                         // :78
+    // x sweep (out of place, so that the fused second half ends in U0)
+    RUN(mhd::launch_fluidx(U1, U0, B0, nx, ny, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :71
+    RUN(mhd::launch_advectbyzx(B1, B0, U1, B0, nx, ny, nz, dt, 0.0f, s, f));               // :74
+    RUN(mhd::launch_transpose12(U0, B0, U1, B1, nx, ny, nz, s));                         // :78
     // y sweep, dims (ny, nx, nz)
-    RUN(mhd::launch_fluidx(U1, U1, B0, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :80
-    RUN(mhd::launch_advectbyzx(B1, B0, U1, B0, ny, nx, nz, dt, 0.0f, s, f));               // :84
-    RUN(mhd::launch_transpose13(U0, B0, U1, B1, ny, nx, nz, s));                         // :88
+    RUN(mhd::launch_fluidx(U0, U0, B0, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :80
+    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, ny, nx, nz, dt, 0.0f, s, f));               // :84
+    RUN(mhd::launch_transpose13(U1, B0, U0, B1, ny, nx, nz, s));                         // :88
     // z sweeps, dims (nz, nx, ny)
-    RUN(mhd::launch_fluidx(U0, U0, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :91
-    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, nz, nx, ny, dt, 0.0f, s, f));               // :93
-    RUN(mhd::launch_advectbyzx(B0, B1, U0, B1, nz, nx, ny, dt, 0.0f, s, f));               // :95
-    RUN(mhd::launch_fluidx(U0, U0, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :97
+    RUN(mhd::launch_fluidx(U1, U1, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :91
+    RUN(mhd::launch_advectbyzx(B1, B0, U1, B0, nz, nx, ny, dt, 0.0f, s, f));               // :93
+    // second half: each advectbyzx + fluidx pair is one fused kernel
+    RUN(mhd::launch_advect_fluid(U0, B0, U1, B1, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr,
+                                 0.0, f));                                               // :95-97
     RUN(mhd::launch_transpose13(U1, B1, U0, B0, nz, nx, ny, s));                         // :102
     // y again, dims (ny, nx, nz)
-    RUN(mhd::launch_advectbyzx(B0, B1, U1, B1, ny, nx, nz, dt, 0.0f, s, f));               // :105
-    RUN(mhd::launch_fluidx(U1, U1, B0, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :107
-    RUN(mhd::launch_transpose12(U0, B1, U1, B0, ny, nx, nz, s));                         // :112
+    RUN(mhd::launch_advect_fluid(U0, B0, U1, B1, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr,
+                                 0.0, f));                                               // :105-107
+    RUN(mhd::launch_transpose12(U1, B1, U0, B0, ny, nx, nz, s));                         // :112
     // x again; its sweep also reduces calcfl of the new state for the next step
-    RUN(mhd::launch_advectbyzx(B0, B1, U0, B1, nx, ny, nz, dt, 0.0f, s, f));               // :115
-    RUN(mhd::launch_fluidx(U0, U0, B0, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl, f));  // :117
+    RUN(mhd::launch_advect_fluid(U0, B0, U1, B1, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl,
+                                 f));                                                    // :115-117
 #undef RUN
     return rc;
 }

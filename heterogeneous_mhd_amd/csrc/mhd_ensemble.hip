@@ -85,8 +85,9 @@ __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout
         const int n = E.n[box];
         const long long bo = E.boff[box];
         __syncthreads();
+        float keep[M][5];
         advect_group<BLOCK, M, FAST>(lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n, n, tp,
-                               item - E.gpre[box], E.dt[box]);
+                                     item - E.gpre[box], E.dt[box], keep);
     }
 }
 

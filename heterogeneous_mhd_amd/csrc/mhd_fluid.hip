@@ -4,7 +4,7 @@
 namespace mhd {
 namespace {
 
-template <int BLOCK, int M, bool CFL, bool FAST>
+template <int BLOCK, int M, bool CFL, bool FAST, int TP, bool FIT>
 __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float *uu,
                                                        const float *__restrict__ bu, int nx, int ny,
                                                        int nz, int tp, const float *__restrict__ dtp,
@@ -15,7 +15,8 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float
     const int pb = BLOCK / tp;
     const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
     const float dt = dtp ? *dtp : dtv;
-    fluid_group<BLOCK, M, CFL, FAST>(lds, red, u_out, uu, bu, nx, ny, nz, tp, grp, dt, co.ws, grp);
+    fluid_group<BLOCK, M, CFL, FAST, false, TP, FIT>(lds, red, u_out, uu, bu, nx, ny, nz, tp, grp, dt,
+                                                     co.ws, grp);
 }
 
 struct Cfg {
@@ -32,20 +33,22 @@ Cfg pick(int nx)
     return {1024, 2, 1024};
 }
 
-template <int BLOCK, int M, bool CFL, bool FAST>
-int launch_cfg(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz, int tp,
+template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+int launch_cfg(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
 {
-    const int pb = BLOCK / tp;
+    constexpr int pb = BLOCK / TP;
     const long npen = (long)ny * nz;
     const long nblk = (npen + pb - 1) / pb;
     const size_t lds = sizeof(float) * 10 * (size_t)nx * pb;
+    // nx == M*TP: the compile-time pencil length (no bounds tests, immediate offsets)
+    auto k = nx == M * TP ? fluidx_kernel<BLOCK, M, CFL, FAST, TP, true>
+                          : fluidx_kernel<BLOCK, M, CFL, FAST, TP, false>;
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)fluidx_kernel<BLOCK, M, CFL, FAST>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
-    hipLaunchKernelGGL((fluidx_kernel<BLOCK, M, CFL, FAST>), dim3((unsigned)nblk), dim3(BLOCK), lds, s,
-                       u_out, uu, bu, nx, ny, nz, tp, dtp, dtv, co);
+    hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(BLOCK), lds, s, u_out, uu, bu, nx, ny, nz, TP, dtp,
+                       dtv, co);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -54,13 +57,14 @@ int launch_any(float *u_out, const float *uu, const float *bu, int nx, int ny, i
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
 {
     const Cfg c = pick(nx);
-    if (c.block == 256 && c.m == 1)
-        return launch_cfg<256, 1, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
-    if (c.block == 256)
-        return launch_cfg<256, 2, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
-    if (c.block == 512)
-        return launch_cfg<512, 2, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
-    return launch_cfg<1024, 2, CFL, FAST>(u_out, uu, bu, nx, ny, nz, c.tp, dtp, dtv, co, s);
+    switch (c.tp * 4 + c.m) {
+    case 64 * 4 + 1: return launch_cfg<256, 1, 64, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
+    case 64 * 4 + 2: return launch_cfg<256, 2, 64, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
+    case 128 * 4 + 2: return launch_cfg<256, 2, 128, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
+    case 256 * 4 + 2: return launch_cfg<256, 2, 256, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
+    case 512 * 4 + 2: return launch_cfg<512, 2, 512, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
+    default: return launch_cfg<1024, 2, 1024, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
+    }
 }
 
 template <bool CFL>

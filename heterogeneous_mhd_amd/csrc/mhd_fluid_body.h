@@ -160,7 +160,7 @@ __device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int n
         const float4 *sa = reinterpret_cast<const float4 *>(a_src);
         const float4 *sb = reinterpret_cast<const float4 *>(b_src);
         const int na4 = na >> 2, nb4 = nb >> 2;
-        float4 ra[KA], rb[KB];
+        float4 ra[KA], rb[KB > 0 ? KB : 1];
 #pragma unroll
         for (int k = 0; k < KA; k++) {
             const int f = threadIdx.x + k * BLOCK;
@@ -187,13 +187,22 @@ __device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int n
     }
 }
 
-template <int BLOCK, int M, bool CFL, bool FAST>
+// FUSED: the face field is not read from bu but taken from the registers of
+// a preceding advectbyzx of the same pencils (keep, see advect_group); the
+// caller has synchronised the workgroup since its last LDS read.
+// TPC / FIT: as for advect_group (compile-time threads per pencil and nx).
+template <int BLOCK, int M, bool CFL, bool FAST, bool FUSED = false, int TPC = 0, bool FIT = false>
 __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out, const float *uu,
-                                            const float *__restrict__ bu, int nx, int ny, int nz,
-                                            int tp, int grp, float dt, void *cfl_ws, int rec)
+                                            const float *__restrict__ bu, int nx_, int ny, int nz,
+                                            int tp_, int grp, float dt, void *cfl_ws, int rec,
+                                            const float (*keep)[5] = nullptr)
 {
+    static_assert(!FIT || TPC > 0, "FIT needs the threads per pencil at compile time");
+    const int tp = TPC > 0 ? TPC : tp_;
+    const int nx = FIT ? M * TPC : nx_;
     const int pb = BLOCK / tp;                 // pencils per workgroup
-    const int lp = threadIdx.x / tp;           // this thread's pencil in the group
+    // this thread's pencil in the group: uniform over the wavefront
+    const int lp = __builtin_amdgcn_readfirstlane(threadIdx.x / tp);
     const int lt = threadIdx.x - lp * tp;      // this thread's rank in its pencil
     const int npen = ny * nz;
     const int pen0 = grp * pb;
@@ -218,15 +227,29 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
     const bool vec = ((uintptr_t)gu % 16 == 0) && ((uintptr_t)gb % 16 == 0) && nu % 4 == 0 &&
                      nb % 4 == 0 && (5 * nx * pb) % 4 == 0;
     float b2jp[M], b3kp[M];
+    if constexpr (FUSED) {
+        stage_in<BLOCK, KU, 0>(A, gu, nu, B, gb, 0, vec);
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            b2jp[r] = bu[(((size_t)k * ny + jp) * nx + i) * 3 + 1];
-            b3kp[r] = bu[(((size_t)kp * ny + j) * nx + i) * 3 + 2];
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) B[3 * nx * lp + 3 * i + c] = keep[r][c];
+                b2jp[r] = keep[r][3];
+                b3kp[r] = keep[r][4];
+            }
         }
+    } else {
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
+                b2jp[r] = bu[(((size_t)k * ny + jp) * nx + i) * 3 + 1];
+                b3kp[r] = bu[(((size_t)kp * ny + j) * nx + i) * 3 + 2];
+            }
+        }
+        stage_in<BLOCK, KU, KB>(A, gu, nu, B, gb, nb, vec);
     }
-    stage_in<BLOCK, KU, KB>(A, gu, nu, B, gb, nb, vec);
     __syncthreads();
 
     float q[M][5], bc[M][3], v[M][5], bf[CFL ? M : 1][3];

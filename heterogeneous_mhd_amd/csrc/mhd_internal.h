@@ -66,6 +66,13 @@ int launch_advectbyzx(float *b_out, const float *b_main, const float *uu, const 
                       int ny, int nz, const float *dt_dev, float dt_val, hipStream_t s,
                       bool fast = false);
 
+// advectbyzx (b_out from bu, b_main = bu) followed by fluidx (u_out from uu
+// and the new b), bitwise equal to the two launches; u_out must not alias uu
+// nor b_out bu.  cfl_ws etc. as for launch_fluidx.
+int launch_advect_fluid(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
+                        int nz, const float *dt_dev, float dt_val, hipStream_t s, void *cfl_ws,
+                        float *inv_c, float *dt_next, double cfl, bool fast);
+
 int launch_transpose12(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                        int nz, hipStream_t s);
 
