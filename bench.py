@@ -1,18 +1,26 @@
 """Benchmark of the plain relaxing-TVD split MHD step on one MI355X.
 
 Workload (default): one 1024^3 periodic FP32 box, seeded random-phase IC with
-velocity amplitude 0.3 and vector-potential amplitude 0.05, CFL 0.7.  A step
-is one full x,y,z,z,y,x split step (calcfl, 6 fluid sweeps, 6 CT field
-sweeps, 4 transposes) replayed from a captured HIP graph; state, t and dt
-stay in HBM.  Prints one JSON line:
+velocity amplitude 0.3 and vector-potential amplitude 0.05, CFL 0.7, 20
+full steps.  A step is one full x,y,z,z,y,x split step (the reference's 16
+operators: calcfl, 6 fluid sweeps, 6 CT field sweeps, 4 transposes; the
+three advectbyzx+fluidx pairs of the second half run as one fused kernel
+each) replayed from a captured HIP graph; state, t and dt stay in HBM.
+`value` is measured in the FP32 tolerance mode (precision fast, checked
+against the reference by tests/test_gpu_fast.py); the bitwise-exact mode is
+timed beside it ("exact").  Prints one JSON line:
 
   value      cell-updates/s = N * steps / wall time of the timed steps
   roofline   the dominant kernel's algorithmic bytes per launch / its mean
-             duration (HIP events on the launch stream) vs 8 TB/s
+             duration (HIP events on the launch stream) vs 8 TB/s; traffic
+             from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+  step_roofline  792 B/cell (the reference decomposition) per step time
+  ensemble   the 96-box batched workload, aggregate cell-updates/s
+  corrector  learned-corrector errors and GPU times on held-out seeds
   cpu_baseline  the oracle restatement (bitwise equal to the reference C
              path) timed on this host's cores on a bounded 256^3 sample
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--precision fast|exact]
 """
 from __future__ import annotations
 
@@ -26,8 +34,14 @@ import numpy as np
 
 PEAK_HBM_GBS = 8000.0
 # compulsory bytes per cell per call at each operator's C-call boundary (FP32)
-ALGO_BYTES = {"fluidx": 52, "advectbyzx": 32, "transpose12": 64, "transpose13": 64, "calcfl": 32}
-CALLS_PER_STEP = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13": 2, "calcfl": 1}
+ALGO_BYTES = {"fluidx": 52, "advectbyzx": 32, "transpose12": 64, "transpose13": 64, "calcfl": 32,
+              "advect_fluidx": 64}
+# launches per step of the plan: the second half's three advectbyzx+fluidx
+# pairs run as one fused kernel, and calcfl is fused into the last of them
+CALLS_PER_STEP = {"fluidx": 3, "advectbyzx": 3, "transpose12": 2, "transpose13": 2,
+                  "advect_fluidx": 3}
+# the reference decomposition the step roofline is priced on (792 B/cell)
+REF_CALLS = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13": 2, "calcfl": 1}
 METRIC = "cell-updates/s & % HBM roofline per 3D step"
 
 
@@ -42,6 +56,7 @@ def kernel_times(box, reps: int, precision: str = "exact"):
     calls = {
         "fluidx": lambda: ops.fluidx(u2, b, nx, ny, nz, dt, u, b, precision=precision),
         "advectbyzx": lambda: ops.advectbyzx(u, b2, nx, ny, nz, dt, u, b, precision=precision),
+        "advect_fluidx": lambda: ops.advect_fluidx(u2, b2, nx, ny, nz, dt, u, b, precision=precision),
         "transpose12": lambda: ops.transpose12(u2, b2, u, b, nx, ny, nz),
         "transpose13": lambda: ops.transpose13(u2, b2, u, b, nx, ny, nz),
         "calcfl": lambda: ops.calcfl_device(u, b, nx, ny, nz),
@@ -118,14 +133,14 @@ def copy_probe(box, reps: int = 3):
     return 64.0 * box.n / s / 1e9
 
 
-def ensemble_bench(nbox: int = 96, steps: int = 50):
+def ensemble_bench(nbox: int = 96, steps: int = 50, precision: str = "exact"):
     """The batched-ensemble workload: nbox periodic boxes with sides drawn by
     seed from {32,48,64,96,128} and per-box amplitudes, autotuned launch shape,
     graph-replayed steps.  Returns aggregate cell-updates/s and details."""
     import torch
     from heterogeneous_mhd_amd import solver
     sides, specs = solver.ensemble_specs(nbox, seed=7)
-    ens = solver.Ensemble(sides, cfl=0.7)
+    ens = solver.Ensemble(sides, cfl=0.7, precision=precision)
     ens.init(specs)
     best_ms = ens.tune(reps=2)
     ens.run(2, graph=True)
@@ -136,17 +151,46 @@ def ensemble_bench(nbox: int = 96, steps: int = 50):
     wall = time.perf_counter() - t0
     t, dt = ens.times()
     out = {"boxes": nbox, "cells": ens.cells, "steps": steps, "value": ens.cells * steps / wall,
-           "unit": "cell-updates/s", "ms_per_step": 1e3 * wall / steps, "config": ens.config(),
+           "unit": "cell-updates/s", "ms_per_step": 1e3 * wall / steps, "precision": precision,
+           "config": ens.config(),
            "tune_best_ms": best_ms, "dt_range": [min(dt), max(dt)]}
     ens.close()
     torch.cuda.empty_cache()
     return out
 
 
+def timed_steps(box, steps: int, warmup: int, graph: bool):
+    import torch
+    box.run(warmup, graph=graph)
+    box.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    box.run(steps, graph=graph)
+    box.sync()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def corrector_bench(seeds: int, steps: int):
+    """The learned-corrector workload: held-out seeds, 4x-downsampled 256^3
+    runs as the judge; errors of plain 64^3, corrected 64^3 and plain 128^3
+    at 50 and `steps` coarse steps, with their GPU times."""
+    from heterogeneous_mhd_amd import corrector
+    if not corrector.WEIGHTS.exists():
+        return None
+    net = corrector.CorrectorNet.load()
+    rows = corrector.evaluate(net, list(range(5000, 5000 + seeds)), steps, checkpoints=(50, steps),
+                              log=lambda m: print(m, file=sys.stderr, flush=True))
+    out = corrector.summarize(rows)
+    out["workload"] = (f"{seeds} held-out seeds, v amp 0.1, A amp 0.05, {steps} coarse steps; "
+                       "error = mean over (rho, m, E, B) of ||x - ref|| / ||ref - mean(ref)||")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--amp-v", type=float, default=0.3)
@@ -156,7 +200,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=5)
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--precision", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--no-exact", action="store_true", help="skip the bitwise-exact leg")
+    ap.add_argument("--no-corrector", action="store_true")
+    ap.add_argument("--corrector-seeds", type=int, default=2)
+    ap.add_argument("--corrector-steps", type=int, default=100)
+    ap.add_argument("--precision", choices=["exact", "fast"], default="fast")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -167,30 +215,29 @@ def main():
     from heterogeneous_mhd_amd import solver
     n = args.n
     N = n ** 3
+    graph = not args.no_graph
     print(f"[bench] {n}^3 box, allocating state", file=sys.stderr, flush=True)
     box = solver.Box(n, n, n, cfl=args.cfl, precision=args.precision)
     box.init(solver.ICSpec(seed=1, amp_v=args.amp_v, amp_a=args.amp_a))
-    graph = not args.no_graph
-    print("[bench] IC ready, warm-up", file=sys.stderr, flush=True)
-    box.run(args.warmup, graph=graph)
-    box.sync()
-    torch.cuda.synchronize()
-    print("[bench] timing", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    box.run(args.steps, graph=graph)
-    box.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    print(f"[bench] IC ready; {args.warmup} warm-up + {args.steps} timed steps ({args.precision})",
+          file=sys.stderr, flush=True)
+    wall = timed_steps(box, args.steps, args.warmup, graph)
     t_sim, dt = box.time()
     ms_per_step = 1e3 * wall / args.steps
     value = N * args.steps / wall
     print(f"[bench] {ms_per_step:.2f} ms/step, t={t_sim:.4f} dt={dt:.4g}", file=sys.stderr, flush=True)
 
     kt = kernel_times(box, args.kernel_reps, args.precision)
-    per_step = {k: kt[k] * CALLS_PER_STEP[k] for k in kt}
+    per_step = {k: kt[k] * CALLS_PER_STEP.get(k, 0) for k in kt}
     dom = max(per_step, key=per_step.get)
     achieved = ALGO_BYTES[dom] * N / kt[dom] / 1e9
-    step_bytes = sum(ALGO_BYTES[k] * CALLS_PER_STEP[k] for k in ALGO_BYTES) * N
+    step_bytes = sum(ALGO_BYTES[k] * REF_CALLS[k] for k in REF_CALLS) * N
+
+    def step_roofline(w):
+        gbs = step_bytes / (w / args.steps) / 1e9
+        return {"algo_bytes_per_cell_step": step_bytes / N, "achieved_GBs": gbs,
+                "frac": gbs / PEAK_HBM_GBS}
+
     diag = box.diagnostics()
     copy_gbs = copy_probe(box)
     line = {
@@ -202,20 +249,30 @@ def main():
                                f"v amp {args.amp_v}, A amp {args.amp_a}, CFL {args.cfl}",
                    "nx": n, "ny": n, "nz": n, "graph": graph, "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": pmc_traffic(dom, N),
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": pmc_traffic(dom + (" fast" if args.precision == "fast" else ""), N),
                      "algo_bytes_per_cell": ALGO_BYTES[dom], "mean_launch_s": kt[dom]},
-        "step_roofline": {"algo_bytes_per_cell_step": step_bytes / N,
-                          "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
-                          "frac": step_bytes / (wall / args.steps) / 1e9 / PEAK_HBM_GBS},
+        "step_roofline": step_roofline(wall),
         "kernel_ms": {k: 1e3 * v for k, v in kt.items()},
+        "kernel_GBs": {k: ALGO_BYTES[k] * N / v / 1e9 for k, v in kt.items()},
         "copy_GBs": copy_gbs,
         "divb_max": diag["divb_max"],
     }
+    if args.precision == "fast" and not args.no_exact:
+        # the bitwise-exact mode on the same box (the state just evolves further)
+        print("[bench] exact leg", file=sys.stderr, flush=True)
+        box.set_precision("exact")
+        we = timed_steps(box, args.steps, 1, graph)
+        line["exact"] = {"value": N * args.steps / we, "ms_per_step": 1e3 * we / args.steps,
+                         "step_roofline": step_roofline(we)}
     del box
     torch.cuda.empty_cache()
     if not args.no_ensemble:
         print("[bench] batched ensemble", file=sys.stderr, flush=True)
-        line["ensemble"] = ensemble_bench()
+        line["ensemble"] = ensemble_bench(precision=args.precision)
+    if not args.no_corrector:
+        print("[bench] learned corrector", file=sys.stderr, flush=True)
+        line["corrector"] = corrector_bench(args.corrector_seeds, args.corrector_steps)
     if not args.no_cpu:
         # the CPU leg steps a 256^3 box of the same generator (the GPU makes
         # the IC, bitwise equal to the host generator's)

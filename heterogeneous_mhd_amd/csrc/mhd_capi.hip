@@ -283,6 +283,16 @@ int mhd_advectbyzx_fast_async(float *b, const float *b_main, const float *update
                                   (hipStream_t)stream, true);
 }
 
+int mhd_advect_fluidx_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
+                            int ny, int nz, const float *dt_dev, float dt, int precision,
+                            mhd_stream_t stream)
+{
+    if (precision != MHD_PREC_EXACT && precision != MHD_PREC_FAST) return MHD_ERR_ARG;
+    return mhd::launch_advect_fluid(u, b, update_u, update_b, nx, ny, nz, dt_dev, dt,
+                                    (hipStream_t)stream, nullptr, nullptr, nullptr, 0.0,
+                                    precision == MHD_PREC_FAST);
+}
+
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream)
 {

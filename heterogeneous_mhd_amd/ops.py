@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import torch
 
-from ._lib import check, lib
+from ._lib import MHD_PREC_EXACT, MHD_PREC_FAST, check, lib
 
 
 def _dev(t: torch.Tensor, n: int, what: str) -> int:
@@ -63,6 +63,19 @@ def advectbyzx(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
     f = lib().mhd_advectbyzx_fast_async if precision == "fast" else lib().mhd_advectbyzx_async
     check(f(pb, pb, _dev(update_u, 5 * n, "update_u"), _dev(update_b, 3 * n, "update_b"), nx, ny,
             nz, dp, dv, _stream()), "advectbyzx")
+
+
+def advect_fluidx(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
+    """advectbyzx then fluidx of the same sweep as one kernel (bitwise equal
+    to the two calls): b <- CT update of update_b, u <- fluid sweep of
+    update_u with the new b.  u, b must not alias update_u, update_b."""
+    n = nx * ny * nz
+    dp, dv = _dt(dt)
+    check(lib().mhd_advect_fluidx_async(_dev(u, 5 * n, "u"), _dev(b, 3 * n, "b"),
+                                        _dev(update_u, 5 * n, "update_u"),
+                                        _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv,
+                                        MHD_PREC_FAST if precision == "fast" else MHD_PREC_EXACT,
+                                        _stream()), "advect_fluidx")
 
 
 def transpose12(u, b, update_u, update_b, nx, ny, nz, iter=0):

@@ -26,12 +26,16 @@ import sys
 
 # kernel-name prefix -> operator whose C call launches it
 _OPS = [
-    (re.compile(r"fluidx_kernel<\d+, \d+, false, false>"), "fluidx"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, false, true>"), "fluidx fast"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, true, false>"), "fluidx+calcfl (fused, last sweep)"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, true, true>"), "fluidx+calcfl fast (fused, last sweep)"),
-    (re.compile(r"advect_kernel<\d+, \d+, false>"), "advectbyzx"),
-    (re.compile(r"advect_kernel<\d+, \d+, true>"), "advectbyzx fast"),
+    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, false[,>]"), "advect_fluidx"),
+    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, true[,>]"), "advect_fluidx fast"),
+    (re.compile(r"advect_fluid_kernel<\d+, \d+, true, false[,>]"), "advect_fluidx+calcfl"),
+    (re.compile(r"advect_fluid_kernel<\d+, \d+, true, true[,>]"), "advect_fluidx+calcfl fast"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, false, false[,>]"), "fluidx"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, false, true[,>]"), "fluidx fast"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, true, false[,>]"), "fluidx+calcfl"),
+    (re.compile(r"fluidx_kernel<\d+, \d+, true, true[,>]"), "fluidx+calcfl fast"),
+    (re.compile(r"advect_kernel<\d+, \d+, false[,>]"), "advectbyzx"),
+    (re.compile(r"advect_kernel<\d+, \d+, true[,>]"), "advectbyzx fast"),
     (re.compile(r"transpose_kernel<5>"), "transpose (u, 5 comps)"),
     (re.compile(r"transpose_kernel<3>"), "transpose (b, 3 comps)"),
     (re.compile(r"cfl_kernel"), "calcfl"),
@@ -92,10 +96,10 @@ def _largest_copy(d):
 
 def stats(path: str) -> None:
     rows = list(csv.DictReader(open(path)))
-    print(f"{'kernel':60s} {'operator':34s} {'calls':>6s} {'mean_ms':>9s} {'min_ms':>9s} "
+    print(f"{'kernel':60s} {'operator':28s} {'calls':>6s} {'mean_ms':>9s} {'min_ms':>9s} "
           f"{'max_ms':>9s} {'total_%':>7s}")
     for r in rows:
-        print(f"{short_name(r['Name']):60s} {str(op_of(r['Name'])):34s} {int(r['Calls']):6d} "
+        print(f"{short_name(r['Name']):60s} {str(op_of(r['Name'])):28s} {int(r['Calls']):6d} "
               f"{float(r['AverageNs']) / 1e6:9.3f} {float(r['MinNs']) / 1e6:9.3f} "
               f"{float(r['MaxNs']) / 1e6:9.3f} {float(r['Percentage']):7.2f}")
 

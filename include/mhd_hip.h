@@ -105,6 +105,14 @@ int mhd_fluidx_fast_async(float *u, const float *update_u, const float *update_b
 int mhd_advectbyzx_fast_async(float *b, const float *b_main, const float *update_u,
                               const float *update_b, int nx, int ny, int nz, const float *dt_dev,
                               float dt, mhd_stream_t stream);
+/* advectbyzx (b from update_b, with b_main = update_b) followed by fluidx
+ * (u from update_u and the new b) in one kernel, bitwise equal to the two
+ * calls -- the pairs at C/mhd.c:95-97, :105-107, :115-117 (no reference
+ * counterpart as one call).  u must not alias update_u, b not update_b.
+ * precision: MHD_PREC_EXACT or MHD_PREC_FAST. */
+int mhd_advect_fluidx_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
+                            int ny, int nz, const float *dt_dev, float dt, int precision,
+                            mhd_stream_t stream);
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream);
 int mhd_transpose13_async(float *u, float *b, const float *update_u, const float *update_b, int nx,

@@ -186,3 +186,26 @@ def test_calcfl_nan_semantics(gpu):
         got = ops.calcfl(_t(uu, gpu), _t(b, gpu), nx, ny, nz)
         want = O.calcfl(uu, b, nx, ny, nz)
         assert np.float32(got).tobytes() == want.tobytes(), pos
+
+
+@pytest.mark.parametrize("case", OPS_CASES)
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_fused_advect_fluidx_equals_the_two_calls(gpu, case, precision):
+    """advect_fluidx is bitwise advectbyzx followed by fluidx (C/mhd.c:95-97)."""
+    import torch
+    from heterogeneous_mhd_amd import ops
+    d = load_ops(case)
+    nx, ny, nz = d["dims"]
+    dt = float(d["dt"])
+    print(f"fused {case} {precision}", flush=True)
+    u, b = _t(d["u"], gpu), _t(d["b"], gpu)
+    b1 = b.clone()
+    ops.advectbyzx(u, b1, nx, ny, nz, dt, u, b, precision=precision)
+    u1 = torch.empty_like(u)
+    ops.fluidx(u1, b1, nx, ny, nz, dt, u, b1, precision=precision)
+    uf, bf = torch.empty_like(u), torch.empty_like(b)
+    ops.advect_fluidx(uf, bf, nx, ny, nz, dt, u, b, precision=precision)
+    assert bits_equal(_h(bf), _h(b1))
+    assert bits_equal(_h(uf), _h(u1))
+    if precision == "exact":
+        assert bits_equal(_h(bf), d["advectbyzx_b"])
