@@ -207,8 +207,8 @@ def main():
     ap.add_argument("--precision", choices=["exact", "fast"], default="fast")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the step would shard)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the z-slab "
+              "decomposition with all-to-all transposes is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch
