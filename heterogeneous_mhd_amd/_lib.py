@@ -111,8 +111,8 @@ SIGNATURES = {
     "mhd_corr_loss_grad": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "mhd_downsample": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "mhd_corr_check_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
-    "mhd_corr_apply": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_double, _vp, _vp,
-                                 _vp, _vp, _vp]),
+    "mhd_corr_apply": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_float,
+                                 C.c_double, _vp, _vp, _vp, _vp, _vp]),
     "mhd_diagnostics": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(C.c_double),
                                   _vp]),
 }

@@ -5,10 +5,12 @@ through the library's own BF16 MFMA kernels (include/mhd_hip.h, Part 3);
 torch only holds device memory and does the data bookkeeping.
 
 Correction after each coarse step, with y the network output on the stepped
-state:  u_c += s_c * (y_c - <y_c>) (c < 5; the box mean is removed because
-the scheme and the fine solution conserve the totals of rho, m and E),
-b += s_B * curl(y_5..7)  -- the field is corrected through a vector
-potential, so div b stays at round-off.  The
+state:  u_c += alpha s_c (y_c - <y_c>) (c < 5; the box mean is removed
+because the scheme and the fine solution conserve the totals of rho, m and
+E), b += alpha s_B curl(y_5..7) -- the field is corrected through a vector
+potential, so div b stays at round-off.  The relaxation alpha (and an
+optional limit on the output values) is chosen on validation seeds by
+`select` and stored with the weights.  The
 correction is skipped (the plain step is kept) when the corrected state has
 rho <= 0 or p <= 0 anywhere or changes the total energy by more than etol.
 
@@ -18,6 +20,7 @@ from that state is compared with the downsampled fine state at the same
 time (fine time = 4 x coarse time, since the scheme has unit cell size).
 
     python -m heterogeneous_mhd_amd.corrector train --out weights.npz [...]
+    python -m heterogeneous_mhd_amd.corrector select --weights weights.npz [...]
     python -m heterogeneous_mhd_amd.corrector evaluate --weights weights.npz [...]
 """
 from __future__ import annotations
@@ -177,10 +180,11 @@ class CorrectorNet:
                                   1.0 / batch, _st()), "corr_adam")
         self.prep()
 
-    def apply(self, buf: Buffers, u, b, y, etol: float, flag_stats: torch.Tensor, scale=None):
+    def apply(self, buf: Buffers, u, b, y, etol: float, flag_stats: torch.Tensor, scale=None,
+              limit: float = 0.0):
         nx, ny, nz = buf.shape
         sc = self.scale if scale is None else scale
-        check(lib().mhd_corr_apply(_p(u), _p(b), _p(y), nx, ny, nz, _p(sc), float(etol),
+        check(lib().mhd_corr_apply(_p(u), _p(b), _p(y), nx, ny, nz, _p(sc), float(limit), float(etol),
                                    _p(buf.u_cand), _p(buf.b_cand), _p(buf.check), _p(flag_stats),
                                    _st()), "corr_apply")
 
@@ -291,7 +295,7 @@ def make_onpolicy_pairs(net: CorrectorNet, seeds, rollout: int, every: int, fine
                 pairs.append((bx.u.clone(), bx.b.clone(), torch.cat([du, db], dim=1).contiguous()))
             with torch.cuda.stream(bx.stream):
                 y = net.forward(cb.buf, bx.u, bx.b)
-                net.apply(cb.buf, bx.u, bx.b, y, cb.etol, cb.stats, cb.scale)
+                net.apply(cb.buf, bx.u, bx.b, y, cb.etol, cb.stats, cb.scale, cb.limit)
             bx.state_changed()
         if log and (si + 1) % 10 == 0:
             log(f"[corrector] on-policy pairs from {si + 1}/{len(seeds)} seeds, "
@@ -368,9 +372,11 @@ class CorrectedBox:
     (with the positivity / energy-drift fallback)."""
 
     def __init__(self, net: CorrectorNet, n: int, etol: float = 1e-3, cfl: float = 0.7,
-                 alpha: float | None = None):
+                 alpha: float | None = None, limit: float | None = None):
         self.net, self.n, self.etol = net, n, etol
         self.alpha = float(net.meta.get("alpha", 1.0) if alpha is None else alpha)
+        # network outputs limited to +-limit (units of the training correction's std dev)
+        self.limit = float(net.meta.get("limit", 0.0) if limit is None else limit)
         self.scale = net.scale * self.alpha  # relaxation of the correction
         self.box = solver.Box(n, n, n, cfl=cfl)
         self.buf = Buffers(n, n, n, net.device, train=False)
@@ -382,7 +388,8 @@ class CorrectedBox:
             bx.run(1, graph=True)
             with torch.cuda.stream(bx.stream):
                 y = self.net.forward(self.buf, bx.u, bx.b)
-                self.net.apply(self.buf, bx.u, bx.b, y, self.etol, self.stats, self.scale)
+                self.net.apply(self.buf, bx.u, bx.b, y, self.etol, self.stats, self.scale,
+                               self.limit)
             bx.state_changed()
 
     def fallbacks(self):
@@ -412,7 +419,7 @@ def _timed(fn):
 
 def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1, 0.1),
              amp_a=(0.05, 0.05), seed: int = 99, etol: float = 1e-3, alpha: float | None = None,
-             checkpoints=None, log=print):
+             checkpoints=None, limit: float | None = None, log=print):
     """Per held-out seed: plain n^3, corrected n^3 (n = fine/4) and plain
     2n^3 runs against the 4x downsampled fine run, at the times the corrected
     run reaches after each checkpoint step count (default: nsteps); returns
@@ -430,7 +437,7 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
             u4, b4 = downsample(fbox.u, fbox.b, fine, 4)
             u2, b2 = downsample(fbox.u, fbox.b, fine, 2)
         fbox.stream.synchronize()
-        cb = CorrectedBox(net, n, etol=etol, alpha=alpha)
+        cb = CorrectedBox(net, n, etol=etol, alpha=alpha, limit=limit)
         cb.box.u.copy_(u4)
         cb.box.b.copy_(b4)
         cb.box.set_time(0.0)
@@ -461,6 +468,7 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
         row.update({k: last[k] for k in ("T", "plain", "corrected", "plain_2x", "gpu_s")})
         row["fallbacks"] = cb.fallbacks()
         row["alpha"] = cb.alpha
+        row["limit"] = cb.limit
         res.append(row)
         if log:
             log(f"[corrector] seed {s}: " + "; ".join(
@@ -477,6 +485,7 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
 def summarize(rows):
     keys = ("plain", "corrected", "plain_2x")
     out = {"seeds": len(rows), "alpha": rows[0]["alpha"] if rows else None,
+           "limit": rows[0]["limit"] if rows else None,
            "err_mean": {k: float(np.mean([r[k]["mean"] for r in rows])) for k in keys},
            "gpu_s_mean": {k: float(np.mean([r["gpu_s"][k] for r in rows])) for k in keys},
            "fallback_steps": int(sum(r["fallbacks"]["rejected"] for r in rows))}
@@ -529,12 +538,23 @@ def main(argv=None):
     f.add_argument("--val-steps", type=int, default=200)
     f.add_argument("--eval-seeds", type=int, default=0)
     f.add_argument("--eval-steps", type=int, default=200)
+    sel = sub.add_parser("select", help="pick the correction relaxation and output limit on "
+                                        "validation seeds 4000.. and store them in the weights")
+    sel.add_argument("--weights", default=str(WEIGHTS))
+    sel.add_argument("--out", default=None)
+    sel.add_argument("--alphas", default="0.25,0.5,1.0")
+    sel.add_argument("--limits", default="0,1.5,3")
+    sel.add_argument("--seeds", type=int, default=2)
+    sel.add_argument("--steps", type=int, default=200)
+    sel.add_argument("--fine", type=int, default=256)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=str(WEIGHTS))
     e.add_argument("--seeds", type=int, default=50)
     e.add_argument("--steps", type=int, default=200)
     e.add_argument("--fine", type=int, default=256)
     e.add_argument("--json", default=None)
+    e.add_argument("--alpha", type=float, default=None)
+    e.add_argument("--limit", type=float, default=None)
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
     if a.cmd == "train":
@@ -603,10 +623,31 @@ def main(argv=None):
             rows = evaluate(net, list(range(5000, 5000 + a.eval_seeds)), a.eval_steps, fine=a.fine,
                             checkpoints=(50, 100, a.eval_steps), log=log)
             log("[corrector] summary " + json.dumps(summarize(rows)))
+    elif a.cmd == "select":
+        net = CorrectorNet.load(a.weights)
+        grid = {}
+        for al in [float(x) for x in a.alphas.split(",") if x]:
+            for lim in [float(x) for x in a.limits.split(",") if x]:
+                rows = evaluate(net, list(range(4000, 4000 + a.seeds)), a.steps, fine=a.fine,
+                                checkpoints=(50, 100, a.steps), alpha=al, limit=lim, log=None)
+                sm = summarize(rows)
+                # mean over the checkpoints: good early and still good late
+                score = float(np.mean([v["corrected"] for v in sm["err_mean_at_steps"].values()]))
+                grid[f"{al},{lim}"] = {"score": score, "err_mean_at_steps": sm["err_mean_at_steps"],
+                                      "fallback_steps": sm["fallback_steps"]}
+                log(f"[corrector] alpha {al} limit {lim}: score {score:.4f} "
+                    f"{sm['err_mean_at_steps']} fallbacks {sm['fallback_steps']}")
+        best = min(grid, key=lambda k: grid[k]["score"])
+        al, lim = (float(x) for x in best.split(","))
+        net.meta.update({"alpha": al, "limit": lim,
+                         "selection": {"seeds": [4000, 4000 + a.seeds - 1], "steps": a.steps,
+                                       "grid": grid}})
+        net.save(a.out or a.weights)
+        log(f"[corrector] selected alpha {al} limit {lim}; saved {a.out or a.weights}")
     else:
         net = CorrectorNet.load(a.weights)
         rows = evaluate(net, list(range(5000, 5000 + a.seeds)), a.steps, fine=a.fine,
-                        checkpoints=(50, 100, a.steps), log=log)
+                        checkpoints=(50, 100, a.steps), alpha=a.alpha, limit=a.limit, log=log)
         summ = summarize(rows)
         log("[corrector] summary " + json.dumps(summ))
         if a.json:

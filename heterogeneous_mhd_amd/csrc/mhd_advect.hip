@@ -40,12 +40,15 @@ template <bool FAST>
 int launch_any(float *b_out, const float *b_main, const float *uu, const float *bu, int nx, int ny,
                int nz, const float *dt_dev, float dt_val, hipStream_t s)
 {
-    if (nx <= 64) return launch_cfg<256, 1, 64, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
-    if (nx <= 128) return launch_cfg<256, 2, 64, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
-    if (nx <= 256) return launch_cfg<256, 2, 128, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
-    if (nx <= 512) return launch_cfg<256, 2, 256, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
-    if (nx <= 1024) return launch_cfg<512, 2, 512, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
-    return launch_cfg<1024, 2, 1024, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    const SweepCfg c = sweep_cfg(nx);
+    switch (c.tp * 4 + c.m) {
+    case 64 * 4 + 1: return launch_cfg<256, 1, 64, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    case 64 * 4 + 2: return launch_cfg<256, 2, 64, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    case 128 * 4 + 2: return launch_cfg<256, 2, 128, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    case 256 * 4 + 2: return launch_cfg<256, 2, 256, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    case 512 * 4 + 2: return launch_cfg<512, 2, 512, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    default: return launch_cfg<1024, 2, 1024, FAST>(b_out, b_main, uu, bu, nx, ny, nz, dt_dev, dt_val, s);
+    }
 }
 
 }  // namespace

@@ -69,6 +69,12 @@ __device__ __forceinline__ float gelu_grad(float z)
 
 enum { kModeHidden = 0, kModeOutput = 1, kModeGrad = 2 };
 
+// the network output limited to [-lim, lim] (lim <= 0: no limit)
+__device__ __forceinline__ float ylim(float y, float lim)
+{
+    return lim > 0.0f ? fminf(fmaxf(y, -lim), lim) : y;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
                                                    const float *__restrict__ bias, int nx, int ny,
@@ -275,14 +281,23 @@ __global__ void pack_kernel(const float *__restrict__ u, const float *__restrict
 }
 
 // b = curl A with A on edges (A1 at (i, j-1/2, k-1/2), ...), forward differences;
-// A is read from channels 5..7 of a [V][ld] array.
+// A is read from channels 5..7 of a [V][ld] array, each value limited to
+// [-lim, lim] when lim > 0.
 __device__ __forceinline__ void curl_at(const float *A, int ld, int i, int j, int k, int nx, int ny,
-                                        int nz, float out[3])
+                                        int nz, float out[3], float lim = 0.0f)
 {
-    const float *a0 = A + vid(i, j, k, nx, ny) * ld + 5;
-    const float *ai = A + vid(wrap(i + 1, nx), j, k, nx, ny) * ld + 5;
-    const float *aj = A + vid(i, wrap(j + 1, ny), k, nx, ny) * ld + 5;
-    const float *ak = A + vid(i, j, wrap(k + 1, nz), nx, ny) * ld + 5;
+    const float *p0 = A + vid(i, j, k, nx, ny) * ld + 5;
+    const float *pi = A + vid(wrap(i + 1, nx), j, k, nx, ny) * ld + 5;
+    const float *pj = A + vid(i, wrap(j + 1, ny), k, nx, ny) * ld + 5;
+    const float *pk = A + vid(i, j, wrap(k + 1, nz), nx, ny) * ld + 5;
+    float a0[3], ai[3], aj[3], ak[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        a0[d] = ylim(p0[d], lim);
+        ai[d] = ylim(pi[d], lim);
+        aj[d] = ylim(pj[d], lim);
+        ak[d] = ylim(pk[d], lim);
+    }
     out[0] = (aj[2] - a0[2]) - (ak[1] - a0[1]);
     out[1] = (ak[0] - a0[0]) - (ai[2] - a0[2]);
     out[2] = (ai[1] - a0[1]) - (aj[0] - a0[0]);
@@ -380,14 +395,14 @@ __global__ void downsample_kernel(const float *__restrict__ u, const float *__re
     for (int d = 0; d < 3; d++) bc[v * 3 + d] = (float)(sb[d] * ia);
 }
 
-// Per block: sums of the u channels of the correction (double).
-__global__ __launch_bounds__(256) void ysum_kernel(const float *__restrict__ y, long n,
+// Per block: sums of the u channels of the (limited) correction (double).
+__global__ __launch_bounds__(256) void ysum_kernel(const float *__restrict__ y, long n, float lim,
                                                    double *__restrict__ part)
 {
     __shared__ double red[5][256];
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
 #pragma unroll
-    for (int c = 0; c < 5; c++) red[c][threadIdx.x] = v < n ? (double)y[v * kC + c] : 0.0;
+    for (int c = 0; c < 5; c++) red[c][threadIdx.x] = v < n ? (double)ylim(y[v * kC + c], lim) : 0.0;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w)
@@ -425,15 +440,16 @@ __global__ __launch_bounds__(256) void ymean_kernel(const double *__restrict__ p
 __global__ void apply_kernel(const float *__restrict__ u, const float *__restrict__ b,
                              const float *__restrict__ y, int nx, int ny, int nz,
                              const float *__restrict__ scale, const float *__restrict__ ymean,
-                             float *__restrict__ uo, float *__restrict__ bo)
+                             float lim, float *__restrict__ uo, float *__restrict__ bo)
 {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (long)nx * ny * nz) return;
     const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
 #pragma unroll
-    for (int c = 0; c < 5; c++) uo[v * 5 + c] = u[v * 5 + c] + scale[c] * (y[v * kC + c] - ymean[c]);
+    for (int c = 0; c < 5; c++)
+        uo[v * 5 + c] = u[v * 5 + c] + scale[c] * (ylim(y[v * kC + c], lim) - ymean[c]);
     float cb[3];
-    curl_at(y, kC, i, j, k, nx, ny, nz, cb);
+    curl_at(y, kC, i, j, k, nx, ny, nz, cb, lim);
 #pragma unroll
     for (int d = 0; d < 3; d++) bo[v * 3 + d] = b[v * 3 + d] + scale[5] * cb[d];
 }
@@ -665,8 +681,8 @@ size_t mhd_corr_check_bytes(int nx, int ny, int nz)
 }
 
 int mhd_corr_apply(float *u, float *b, const float *y, int nx, int ny, int nz, const float *scale,
-                   double etol, float *u_cand, float *b_cand, void *check_ws, int *flag_stats,
-                   mhd_stream_t stream)
+                   float limit, double etol, float *u_cand, float *b_cand, void *check_ws,
+                   int *flag_stats, mhd_stream_t stream)
 {
     if (!u || !b || !y || !scale || !u_cand || !b_cand || !check_ws || !flag_stats || nx < 1 ||
         ny < 1 || nz < 1)
@@ -677,10 +693,10 @@ int mhd_corr_apply(float *u, float *b, const float *y, int nx, int ny, int nz, c
     double *part = (double *)check_ws;  // 5 doubles per block, reused by the check
     float *ymean = (float *)(part + 5 * (size_t)nb);
     (void)hipGetLastError();
-    hipLaunchKernelGGL(ysum_kernel, dim3(nb), dim3(256), 0, s, y, n, part);
+    hipLaunchKernelGGL(ysum_kernel, dim3(nb), dim3(256), 0, s, y, n, limit, part);
     hipLaunchKernelGGL(ymean_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, n, ymean);
     hipLaunchKernelGGL(apply_kernel, dim3(nb), dim3(256), 0, s, u, b, y, nx, ny, nz, scale, ymean,
-                       u_cand, b_cand);
+                       limit, u_cand, b_cand);
     hipLaunchKernelGGL(check_kernel, dim3(nb), dim3(256), 0, s, u, u_cand, b_cand, nx, ny, nz, part);
     hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, etol, flag_stats,
                        flag_stats + 1);

@@ -19,20 +19,6 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float
                                                      co.ws, grp);
 }
 
-struct Cfg {
-    int block, m, tp;
-};
-
-Cfg pick(int nx)
-{
-    if (nx <= 64) return {256, 1, 64};
-    if (nx <= 128) return {256, 2, 64};
-    if (nx <= 256) return {256, 2, 128};
-    if (nx <= 512) return {256, 2, 256};
-    if (nx <= 1024) return {512, 2, 512};
-    return {1024, 2, 1024};
-}
-
 template <int BLOCK, int M, int TP, bool CFL, bool FAST>
 int launch_cfg(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
@@ -56,7 +42,7 @@ template <bool CFL, bool FAST>
 int launch_any(float *u_out, const float *uu, const float *bu, int nx, int ny, int nz,
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
 {
-    const Cfg c = pick(nx);
+    const SweepCfg c = sweep_cfg(nx);
     switch (c.tp * 4 + c.m) {
     case 64 * 4 + 1: return launch_cfg<256, 1, 64, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
     case 64 * 4 + 2: return launch_cfg<256, 2, 64, CFL, FAST>(u_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
@@ -79,7 +65,7 @@ int launch_mode(bool fast, float *u_out, const float *uu, const float *bu, int n
 
 size_t fluid_workgroups(int nx, long npen)
 {
-    const Cfg c = pick(nx);
+    const SweepCfg c = sweep_cfg(nx);
     const int pb = c.block / c.tp;
     return (size_t)((npen + pb - 1) / pb);
 }

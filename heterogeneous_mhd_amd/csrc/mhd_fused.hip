@@ -35,21 +35,6 @@ __global__ __launch_bounds__(BLOCK) void advect_fluid_kernel(float *u_out, float
                                                     co.ws, grp, keep);
 }
 
-struct Cfg {
-    int block, m, tp;
-};
-
-// the launch shapes of fluidx and advectbyzx (they agree)
-Cfg pick(int nx)
-{
-    if (nx <= 64) return {256, 1, 64};
-    if (nx <= 128) return {256, 2, 64};
-    if (nx <= 256) return {256, 2, 128};
-    if (nx <= 512) return {256, 2, 256};
-    if (nx <= 1024) return {512, 2, 512};
-    return {1024, 2, 1024};
-}
-
 template <int BLOCK, int M, int TP, bool CFL, bool FAST>
 int launch_cfg(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny, int nz,
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
@@ -72,7 +57,7 @@ template <bool CFL, bool FAST>
 int launch_any(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny, int nz,
                const float *dtp, float dtv, const CflOut &co, hipStream_t s)
 {
-    const Cfg c = pick(nx);
+    const SweepCfg c = sweep_cfg(nx);
     switch (c.tp * 4 + c.m) {
     case 64 * 4 + 1: return launch_cfg<256, 1, 64, CFL, FAST>(u_out, b_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);
     case 64 * 4 + 2: return launch_cfg<256, 2, 64, CFL, FAST>(u_out, b_out, uu, bu, nx, ny, nz, dtp, dtv, co, s);

@@ -48,6 +48,22 @@ __device__ __forceinline__ int sweep_group(int bid, int G, int gj, int nz)
     return s;
 }
 
+// Launch shape of the pencil sweeps (fluidx, advectbyzx and the fused pair
+// share it, so their workgroup -> pencil-group maps and CFL record counts
+// agree): threads per workgroup, cells per thread, threads per pencil.
+struct SweepCfg {
+    int block, m, tp;
+};
+inline SweepCfg sweep_cfg(int nx)
+{
+    if (nx <= 64) return {256, 1, 64};
+    if (nx <= 128) return {256, 2, 64};
+    if (nx <= 256) return {256, 2, 128};
+    if (nx <= 512) return {256, 2, 256};
+    if (nx <= 1024) return {512, 2, 512};
+    return {1024, 2, 1024};
+}
+
 // fast selects the FP32 tolerance mode of the sweep kernels (hardware
 // reciprocal and square root, FMA); otherwise results are bitwise the
 // reference's.

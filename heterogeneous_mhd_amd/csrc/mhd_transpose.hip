@@ -5,16 +5,20 @@
 namespace mhd {
 namespace {
 
+// plane_fast: blockIdx.x runs over the planes and (y, z) over the tiles, so
+// the workgroups in flight share tile rows; transpose13, whose rows lie a
+// whole x-y plane apart in both arrays, uses it.
 template <int NC>
 __global__ __launch_bounds__(kTB) void transpose_kernel(float *__restrict__ out,
                                                        const float *__restrict__ in, int ni,
                                                        int na, size_t sa_in, size_t sz_in,
                                                        size_t si_out, size_t sz_out, Perm perm,
-                                                       int aligned)
+                                                       int aligned, int plane_fast)
 {
     __shared__ __attribute__((aligned(16))) float tile[kT * lds_row<NC>()];
-    const int i0 = blockIdx.x * kT, a0 = blockIdx.y * kT;
-    const size_t zz = blockIdx.z;
+    const int i0 = (plane_fast ? blockIdx.y : blockIdx.x) * kT;
+    const int a0 = (plane_fast ? blockIdx.z : blockIdx.y) * kT;
+    const size_t zz = plane_fast ? blockIdx.x : blockIdx.z;
     const int ti = ni - i0 < kT ? ni - i0 : kT;   // cells present along i
     const int ta = na - a0 < kT ? na - a0 : kT;   // cells present along a
     if (aligned && ti == kT && ta == kT)
@@ -25,7 +29,8 @@ __global__ __launch_bounds__(kTB) void transpose_kernel(float *__restrict__ out,
 
 template <int NC>
 int launch_one(float *out, const float *in, int ni, int na, int nz_planes, size_t sa_in,
-               size_t sz_in, size_t si_out, size_t sz_out, const int *p, hipStream_t s)
+               size_t sz_in, size_t si_out, size_t sz_out, const int *p, hipStream_t s,
+               bool plane_fast = false)
 {
     if (nz_planes > 65535) return MHD_ERR_ARG;
     Perm perm = {};
@@ -36,10 +41,12 @@ int launch_one(float *out, const float *in, int ni, int na, int nz_planes, size_
     // 16-byte vectors need every row start 16-byte aligned: strides multiple of 4 cells
     const int aligned = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0) && sa_in % 4 == 0 &&
                         sz_in % 4 == 0 && si_out % 4 == 0 && sz_out % 4 == 0;
-    dim3 grid((ni + kT - 1) / kT, (na + kT - 1) / kT, nz_planes);
+    const unsigned ti = (ni + kT - 1) / kT, ta = (na + kT - 1) / kT;
+    if (plane_fast && (ti > 65535 || ta > 65535)) return MHD_ERR_ARG;
+    const dim3 grid = plane_fast ? dim3(nz_planes, ti, ta) : dim3(ti, ta, nz_planes);
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
     hipLaunchKernelGGL((transpose_kernel<NC>), grid, dim3(kTB), 0, s, out, in, ni, na, sa_in,
-                       sz_in, si_out, sz_out, perm, aligned);
+                       sz_in, si_out, sz_out, perm, aligned, (int)plane_fast);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -67,9 +74,9 @@ int launch_transpose13(float *u_out, float *b_out, const float *uu, const float 
     // out dims (nz, ny, nx): out cell = z + y*nz + i*nz*ny.
     const size_t sa_in = (size_t)nx * ny, sz_in = (size_t)nx;
     const size_t si_out = (size_t)nz * ny, sz_out = (size_t)nz;
-    int rc = launch_one<5>(u_out, uu, nx, nz, ny, sa_in, sz_in, si_out, sz_out, kPermU13, s);
+    int rc = launch_one<5>(u_out, uu, nx, nz, ny, sa_in, sz_in, si_out, sz_out, kPermU13, s, true);
     if (rc) return rc;
-    return launch_one<3>(b_out, bu, nx, nz, ny, sa_in, sz_in, si_out, sz_out, kPermB13, s);
+    return launch_one<3>(b_out, bu, nx, nz, ny, sa_in, sz_in, si_out, sz_out, kPermB13, s, true);
 }
 
 }  // namespace mhd

@@ -274,17 +274,18 @@ int mhd_corr_loss_grad(const float *y, const float *target, int nx, int ny, int 
  * over the r^2 fine faces of the coarse face (div b stays 0). */
 int mhd_downsample(const float *u, const float *b, int nx, int ny, int nz, int r, float *uc,
                    float *bc, mhd_stream_t stream);
-/* Applies the correction y (FP32 [V][32] output layer) to (u, b) -- the u
- * part projected to zero box total, as the scheme conserves the totals of
- * rho, m and E -- unless the corrected state fails a check: rho > 0 and p > 0 in every cell and
+/* Applies the correction y (FP32 [V][32] output layer) to (u, b) -- each
+ * output value limited to [-limit, limit] when limit > 0, the u part
+ * projected to zero box total, as the scheme conserves the totals of rho, m
+ * and E -- unless the corrected state fails a check: rho > 0 and p > 0 in every cell and
  * |sum E - sum E_before| <= etol * |sum E_before| (FP64 sums).  u_cand,
  * b_cand: scratch of the state's sizes; check_ws: mhd_corr_check_bytes();
  * flag_stats: 3 device ints: [0] = this call accepted, [1] += accepted,
  * [2] += rejected.  Entirely on the device (graph-capturable). */
 size_t mhd_corr_check_bytes(int nx, int ny, int nz);
 int mhd_corr_apply(float *u, float *b, const float *y, int nx, int ny, int nz, const float *scale,
-                   double etol, float *u_cand, float *b_cand, void *check_ws, int *flag_stats,
-                   mhd_stream_t stream);
+                   float limit, double etol, float *u_cand, float *b_cand, void *check_ws,
+                   int *flag_stats, mhd_stream_t stream);
 
 #ifdef __cplusplus
 }

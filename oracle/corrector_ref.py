@@ -176,10 +176,13 @@ def loss_and_grad(y, target):
     return loss, g
 
 
-def apply_correction(u, b, y, scale, etol, gamma=5.0 / 3.0):
+def apply_correction(u, b, y, scale, etol, limit=0.0, gamma=5.0 / 3.0):
     """(u', b', accepted): the corrected state if it passes the positivity
-    and energy checks, else the input state.  The u part of the correction is
-    projected to zero box total (the scheme conserves the totals of u)."""
+    and energy checks, else the input state.  Each output value is limited to
+    [-limit, limit] (limit > 0); the u part of the correction is projected to
+    zero box total (the scheme conserves the totals of u)."""
+    if limit > 0:
+        y = np.clip(y, -limit, limit)
     yu = y[..., :5] - y[..., :5].mean(axis=(0, 1, 2))
     uc = u + scale[:5] * yu
     bcand = b + scale[5] * curl(y[..., 5:8])

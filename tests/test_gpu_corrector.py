@@ -197,6 +197,14 @@ def test_apply_accepts_and_falls_back(gpu):
         bg = bd.cpu().numpy().reshape(nz, ny, nx, 3)
         assert np.allclose(ug, ur, rtol=1e-6, atol=1e-7) and np.allclose(bg, br, rtol=1e-6, atol=1e-7)
     assert stats.cpu().tolist()[1:] == [1, 2]
+    # with the output limit active
+    ud, bd, yd = _dev(u.reshape(-1)), _dev(b.reshape(-1)), _dev(y.reshape(-1))
+    net.apply(buf, ud, bd, yd, 1e-2, stats, limit=0.005)
+    ur, br, ok = R.apply_correction(u.astype(np.float64), b.astype(np.float64), y.astype(np.float64),
+                                    scale, 1e-2, limit=0.005)
+    assert ok and int(stats[0]) == 1
+    assert np.allclose(ud.cpu().numpy().reshape(nz, ny, nx, 5), ur, rtol=1e-6, atol=1e-7)
+    assert np.allclose(bd.cpu().numpy().reshape(nz, ny, nx, 3), br, rtol=1e-6, atol=1e-7)
 
 
 def test_advance_to_lands_on_the_time(gpu):
