@@ -176,16 +176,20 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
 #pragma unroll
             for (int t = 0; t < kLines; t++) {
                 const float *bb = X + t * nx, *vv = Y + t * nx;
-                float w, wp, wm;
+                float w;
                 if constexpr (FAST) {
+                    // the upwind pair {wm, wp} is {(w1-w0)/2, (w-wa)/2} for vh > 0 and
+                    // {(w0-w1)/2, (w-wa)/2} otherwise; the limiter is symmetric and
+                    // homogeneous, so dw = limiter(+-(w1-w0), w-wa)/2
                     const bool up = vh[r][t] > 0.0f;
                     const float w0 = vv[i] * bb[i], w1 = vv[ip] * bb[ip];
-                    const float wa = up ? vv[im] * bb[im] : vv[ipp] * bb[ipp];
+                    const int ia = up ? im : ipp;
+                    const float wa = vv[ia] * bb[ia];
                     w = up ? w0 : w1;
-                    wp = 0.5f * (up ? w1 - w0 : w1 - wa);
-                    wm = 0.5f * (up ? w0 - wa : w0 - w1);
-                    fx[r][t] = (w + limiter_fast(wm, wp)) * dt;
+                    const float d1 = w1 - w0;
+                    fx[r][t] = fmaf(0.5f, limiter_fast(up ? d1 : -d1, w - wa), w) * dt;
                 } else {
+                    float wp, wm;
                     if (vh[r][t] > 0.0f) {
                         w = vv[i] * bb[i];
                         wp = (float)((double)(vv[ip] * bb[ip] - w) / 2.0);

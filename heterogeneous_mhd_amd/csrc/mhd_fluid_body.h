@@ -120,31 +120,42 @@ __device__ __forceinline__ float div_c(float x, float c, float r, bool fast)
     return q;
 }
 
+// One DPP lane exchange of a 32-bit value (CTRL: a gfx9 DPP control code).
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v)
+{
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
 // Max over the threads of one pencil.  x >= 0 and never NaN (the callers
 // start from 0 and keep the running max with a strict '>', as find_max does,
-// so a NaN candidate never enters).  Ends with a workgroup barrier, which the
-// caller relies on to recycle the LDS stencil arrays.
+// so a NaN candidate never enters), so the bit patterns order as unsigned
+// integers and the max is exact in any order: a DPP butterfly within each
+// row of 16 lanes (quad swaps, half-row and row mirrors), two lane swaps
+// across rows, and, when a pencil spans several wavefronts, one LDS
+// exchange.  Ends with a workgroup barrier, which the caller relies on to
+// recycle the LDS stencil arrays.
 template <int BLOCK>
-__device__ __forceinline__ float pencil_max(float x, int tp, float *red)
+__device__ __forceinline__ float pencil_max(float xf, int tp, float *red)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        float o = __shfl_xor(x, off, 64);
-        x = o > x ? o : x;
-    }
+    unsigned x = __float_as_uint(xf);
+    x = max(x, dpp_u32<0xB1>(x));   // quad_perm [1,0,3,2]
+    x = max(x, dpp_u32<0x4E>(x));   // quad_perm [2,3,0,1]
+    x = max(x, dpp_u32<0x141>(x));  // row_half_mirror
+    x = max(x, dpp_u32<0x140>(x));  // row_mirror
+    x = max(x, (unsigned)__shfl_xor((int)x, 16, 64));
+    x = max(x, (unsigned)__shfl_xor((int)x, 32, 64));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (tp > 64) {
-        if (lane == 0) red[wave] = x;
+        unsigned *r = reinterpret_cast<unsigned *>(red);
+        if (lane == 0) r[wave] = x;
         __syncthreads();
         const int wpp = tp >> 6, first = (threadIdx.x / tp) * wpp;
-        x = red[first];
-        for (int w = 1; w < wpp; w++) {
-            float o = red[first + w];
-            x = o > x ? o : x;
-        }
+        x = r[first];
+        for (int w = 1; w < wpp; w++) x = max(x, r[first + w]);
     }
     __syncthreads();
-    return x;
+    return __uint_as_float(x);
 }
 
 // Copies na floats a_src -> a_dst and nb floats b_src -> b_dst (global ->
