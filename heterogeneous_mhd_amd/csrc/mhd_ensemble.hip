@@ -56,9 +56,9 @@ __device__ __forceinline__ int find_box(const int *pre, int nbox, int item)
     return lo;
 }
 
-template <int BLOCK, int M, bool CFL, bool FAST>
-__global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *U, const float *Bf,
-                                                          int tp, void *ws)
+template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+__global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *Uout, const float *Uin,
+                                                          const float *Bf, void *ws)
 {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float red[2 * (BLOCK / 64)];
@@ -66,17 +66,17 @@ __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *U, co
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const int box = find_box(E.gpre, E.nbox, item);
         const int n = E.n[box];
-        float *u = U + E.uoff[box];
         __syncthreads();  // the previous item's last LDS reads are done
         // record index = item: each box's records are contiguous, in memory order
-        fluid_group<BLOCK, M, CFL, FAST>(lds, red, u, u, Bf + E.boff[box], n, n, n, tp,
-                                         item - E.gpre[box], E.dt[box], ws, item);
+        fluid_group<BLOCK, M, CFL, FAST, false, TP>(lds, red, Uout + E.uoff[box], Uin + E.uoff[box],
+                                                    Bf + E.boff[box], n, n, n, TP,
+                                                    item - E.gpre[box], E.dt[box], ws, item);
     }
 }
 
-template <int BLOCK, int M, bool FAST>
+template <int BLOCK, int M, int TP, bool FAST>
 __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout, const float *Bmain,
-                                                           const float *U, const float *Bin, int tp)
+                                                           const float *U, const float *Bin)
 {
     extern __shared__ float lds[];
     const int total = E.gpre[E.nbox];
@@ -86,8 +86,32 @@ __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout
         const long long bo = E.boff[box];
         __syncthreads();
         float keep[M][5];
-        advect_group<BLOCK, M, FAST>(lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n, n, tp,
-                                     item - E.gpre[box], E.dt[box], keep);
+        advect_group<BLOCK, M, FAST, TP>(lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n,
+                                         n, TP, item - E.gpre[box], E.dt[box], keep);
+    }
+}
+
+// advectbyzx + fluidx of one sweep in one pass (see mhd_fused.hip); u goes
+// to Uout, which must not be Uin
+template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+__global__ __launch_bounds__(BLOCK) void ens_fused_kernel(EnsDev E, float *Uout, float *Bout,
+                                                          const float *Uin, const float *Bin, void *ws)
+{
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    __shared__ float red[2 * (BLOCK / 64)];
+    const int total = E.gpre[E.nbox];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int box = find_box(E.gpre, E.nbox, item);
+        const int n = E.n[box];
+        const long long bo = E.boff[box], uo = E.uoff[box];
+        const int grp = item - E.gpre[box];
+        __syncthreads();  // the previous item's last LDS reads are done
+        float keep[M][5];
+        advect_group<BLOCK, M, FAST, TP>(lds, Bout + bo, Bin + bo, Uin + uo, Bin + bo, n, n, n, TP,
+                                         grp, E.dt[box], keep);
+        __syncthreads();
+        fluid_group<BLOCK, M, CFL, FAST, true, TP>(lds, red, Uout + uo, Uin + uo, Bin + bo, n, n, n, TP,
+                                                   grp, E.dt[box], ws, item, keep);
     }
 }
 
@@ -165,80 +189,84 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kGridMult[] = {2, 4, 8, 16};  // persistent workgroups per CU
 constexpr int kNumGrids = sizeof(kGridMult) / sizeof(kGridMult[0]);
 
-template <int BLOCK, int M, bool CFL, bool FAST>
-int launch_fluid_k(const EnsDev &E, float *U, const float *B, int tp, size_t lds, void *ws, int grid,
-                   hipStream_t s)
+template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+int launch_fluid_k(const EnsDev &E, float *Uo, const float *Ui, const float *B, size_t lds, void *ws,
+                   int grid, hipStream_t s)
 {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, CFL, FAST>,
+        (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, TP, CFL, FAST>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
-    hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
-                       U, B, tp, ws);
+    hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, TP, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
+                       Uo, Ui, B, ws);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
-template <int BLOCK, int M>
-int launch_fluid_shape(const EnsDev &E, float *U, const float *B, int tp, int nmax, bool cfl,
-                       bool fast, void *ws, int grid, hipStream_t s)
-{
-    const size_t lds = sizeof(float) * 10 * (size_t)nmax * (BLOCK / tp);
-    if (cfl)
-        return fast ? launch_fluid_k<BLOCK, M, true, true>(E, U, B, tp, lds, ws, grid, s)
-                    : launch_fluid_k<BLOCK, M, true, false>(E, U, B, tp, lds, ws, grid, s);
-    return fast ? launch_fluid_k<BLOCK, M, false, true>(E, U, B, tp, lds, ws, grid, s)
-                : launch_fluid_k<BLOCK, M, false, false>(E, U, B, tp, lds, ws, grid, s);
-}
-
-template <int BLOCK, int M, bool FAST>
+template <int BLOCK, int M, int TP, bool FAST>
 int launch_advect_k(const EnsDev &E, float *Bo, const float *Bm, const float *U, const float *Bi,
-                    int tp, size_t lds, int grid, hipStream_t s)
+                    size_t lds, int grid, hipStream_t s)
 {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)ens_advect_kernel<BLOCK, M, FAST>,
+        (void)hipFuncSetAttribute((const void *)ens_advect_kernel<BLOCK, M, TP, FAST>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
-    hipLaunchKernelGGL((ens_advect_kernel<BLOCK, M, FAST>), dim3(grid), dim3(BLOCK), lds, s, E, Bo,
-                       Bm, U, Bi, tp);
+    hipLaunchKernelGGL((ens_advect_kernel<BLOCK, M, TP, FAST>), dim3(grid), dim3(BLOCK), lds, s, E, Bo,
+                       Bm, U, Bi);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
-template <int BLOCK, int M>
-int launch_advect_shape(const EnsDev &E, float *Bo, const float *Bm, const float *U, const float *Bi,
-                        int tp, int nmax, bool fast, int grid, hipStream_t s)
+template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+int launch_fused_k(const EnsDev &E, float *Uo, float *Bo, const float *Ui, const float *Bi, size_t lds,
+                   void *ws, int grid, hipStream_t s)
 {
-    const size_t lds = sizeof(float) * 12 * (size_t)nmax * (BLOCK / tp);
-    return fast ? launch_advect_k<BLOCK, M, true>(E, Bo, Bm, U, Bi, tp, lds, grid, s)
-                : launch_advect_k<BLOCK, M, false>(E, Bo, Bm, U, Bi, tp, lds, grid, s);
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)ens_fused_kernel<BLOCK, M, TP, CFL, FAST>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL((ens_fused_kernel<BLOCK, M, TP, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
+                       Uo, Bo, Ui, Bi, ws);
+    return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
-int launch_fluid(int shape, bool fast, const EnsDev &E, float *U, const float *B, int nmax, bool cfl,
-                 void *ws, int grid, hipStream_t s)
+// One sweep kernel of one launch shape.  kind 0: fluid (Uo <- Ui, B0);
+// 1: advect (B0 -> Bo with b_main Bm, u U); 2: fused (Uo, Bo <- Ui, Bi).
+template <int BLOCK, int M, int TP>
+int launch_sweep(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo, const float *Ui,
+                 const float *Bi, const float *Bm, int nmax, void *ws, int grid, hipStream_t s)
 {
-    const Shape &c = kShapes[shape];
-    switch (shape) {
-    case 0: return launch_fluid_shape<256, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
-    case 1: return launch_fluid_shape<128, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
-    case 2: return launch_fluid_shape<256, 1>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
-    case 3: return launch_fluid_shape<512, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
-    case 4: return launch_fluid_shape<256, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
-    case 5: return launch_fluid_shape<512, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
-    default: return launch_fluid_shape<1024, 2>(E, U, B, c.tp, nmax, cfl, fast, ws, grid, s);
+    constexpr int pb = BLOCK / TP;
+    if (kind == 1) {
+        const size_t lds = sizeof(float) * 12 * (size_t)nmax * pb;
+        return fast ? launch_advect_k<BLOCK, M, TP, true>(E, Bo, Bm, Ui, Bi, lds, grid, s)
+                    : launch_advect_k<BLOCK, M, TP, false>(E, Bo, Bm, Ui, Bi, lds, grid, s);
     }
+    if (kind == 0) {
+        const size_t lds = sizeof(float) * 10 * (size_t)nmax * pb;
+        if (cfl)
+            return fast ? launch_fluid_k<BLOCK, M, TP, true, true>(E, Uo, Ui, Bi, lds, ws, grid, s)
+                        : launch_fluid_k<BLOCK, M, TP, true, false>(E, Uo, Ui, Bi, lds, ws, grid, s);
+        return fast ? launch_fluid_k<BLOCK, M, TP, false, true>(E, Uo, Ui, Bi, lds, ws, grid, s)
+                    : launch_fluid_k<BLOCK, M, TP, false, false>(E, Uo, Ui, Bi, lds, ws, grid, s);
+    }
+    const size_t lds = sizeof(float) * 12 * (size_t)nmax * pb;
+    if (cfl)
+        return fast ? launch_fused_k<BLOCK, M, TP, true, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s)
+                    : launch_fused_k<BLOCK, M, TP, true, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s);
+    return fast ? launch_fused_k<BLOCK, M, TP, false, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s)
+                : launch_fused_k<BLOCK, M, TP, false, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s);
 }
 
-int launch_advect(int shape, bool fast, const EnsDev &E, float *Bo, const float *Bm, const float *U,
-                  const float *Bi, int nmax, int grid, hipStream_t s)
+int sweep(int shape, int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo,
+          const float *Ui, const float *Bi, const float *Bm, int nmax, void *ws, int grid, hipStream_t s)
 {
-    const Shape &c = kShapes[shape];
-    switch (shape) {
-    case 0: return launch_advect_shape<256, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
-    case 1: return launch_advect_shape<128, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
-    case 2: return launch_advect_shape<256, 1>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
-    case 3: return launch_advect_shape<512, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
-    case 4: return launch_advect_shape<256, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
-    case 5: return launch_advect_shape<512, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
-    default: return launch_advect_shape<1024, 2>(E, Bo, Bm, U, Bi, c.tp, nmax, fast, grid, s);
+    switch (shape) {  // the entries of kShapes
+    case 0: return launch_sweep<256, 2, 64>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
+    case 1: return launch_sweep<128, 2, 64>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
+    case 2: return launch_sweep<256, 1, 128>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
+    case 3: return launch_sweep<512, 2, 64>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
+    case 4: return launch_sweep<256, 2, 128>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
+    case 5: return launch_sweep<512, 2, 256>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
+    default: return launch_sweep<1024, 2, 512>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
     }
 }
 
@@ -319,22 +347,21 @@ int ens_step(mhd_ensemble *e)
     (void)hipGetLastError();
     hipLaunchKernelGGL(ens_begin_kernel, dim3((e->nbox + 63) / 64), dim3(64), 0, s, E, e->tf);  // :63-64
     RUN(ens_error(hipGetLastError()));
-    RUN(launch_fluid(sh, f, E, U0, B0, nm, false, nullptr, g, s));            // :71
-    RUN(launch_advect(sh, f, E, B1, B0, U0, B0, nm, g, s));                   // :74
-    RUN(launch_transpose(E, U1, B0, U0, B1, false, g, s));                 // :78
-    RUN(launch_fluid(sh, f, E, U1, B0, nm, false, nullptr, g, s));            // :80
-    RUN(launch_advect(sh, f, E, B1, B0, U1, B0, nm, g, s));                   // :84
-    RUN(launch_transpose(E, U0, B0, U1, B1, true, g, s));                  // :88
-    RUN(launch_fluid(sh, f, E, U0, B0, nm, false, nullptr, g, s));            // :91
-    RUN(launch_advect(sh, f, E, B1, B0, U0, B0, nm, g, s));                   // :93
-    RUN(launch_advect(sh, f, E, B0, B1, U0, B1, nm, g, s));                   // :95
-    RUN(launch_fluid(sh, f, E, U0, B0, nm, false, nullptr, g, s));            // :97
-    RUN(launch_transpose(E, U1, B1, U0, B0, true, g, s));                  // :102
-    RUN(launch_advect(sh, f, E, B0, B1, U1, B1, nm, g, s));                   // :105
-    RUN(launch_fluid(sh, f, E, U1, B0, nm, false, nullptr, g, s));            // :107
-    RUN(launch_transpose(E, U0, B1, U1, B0, false, g, s));                 // :112
-    RUN(launch_advect(sh, f, E, B0, B1, U0, B1, nm, g, s));                   // :115
-    RUN(launch_fluid(sh, f, E, U0, B0, nm, true, e->ws, g, s));               // :117 + next calcfl
+    // the order of C/mhd.c:71-117; the second half's advectbyzx + fluidx pairs
+    // are fused, and u ping-pongs so that the step ends in (U0, B0)
+    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U0, B0, nullptr, nm, nullptr, g, s));   // :71
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, nullptr, g, s));        // :74
+    RUN(launch_transpose(E, U0, B0, U1, B1, false, g, s));                             // :78
+    RUN(sweep(sh, 0, false, f, E, U0, nullptr, U0, B0, nullptr, nm, nullptr, g, s));   // :80
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U0, B0, B0, nm, nullptr, g, s));        // :84
+    RUN(launch_transpose(E, U1, B0, U0, B1, true, g, s));                              // :88
+    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U1, B0, nullptr, nm, nullptr, g, s));   // :91
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, nullptr, g, s));        // :93
+    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, nullptr, g, s));        // :95-97
+    RUN(launch_transpose(E, U1, B1, U0, B0, true, g, s));                              // :102
+    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, nullptr, g, s));        // :105-107
+    RUN(launch_transpose(E, U1, B1, U0, B0, false, g, s));                             // :112
+    RUN(sweep(sh, 2, true, f, E, U0, B0, U1, B1, nullptr, nm, e->ws, g, s));           // :115-117 + next calcfl
     if (rc == MHD_OK) {
         hipLaunchKernelGGL(ens_cfl_fold_kernel, dim3(e->nbox), dim3(kFoldBlock), 0, s, E, e->ws, e->cfl);
         rc = ens_error(hipGetLastError());
