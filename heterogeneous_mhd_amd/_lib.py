@@ -24,12 +24,13 @@ MHD_ERR_SHAPE = -2
 MHD_ERR_HIP = -3
 MHD_ERR_TOO_LONG = -4
 MHD_ERR_NOMEM = -5
+MHD_ERR_IO = -6
 MHD_PREC_EXACT = 0
 MHD_PREC_FAST = 1
 
 _ERRS = {MHD_ERR_ARG: "bad argument", MHD_ERR_SHAPE: "unsupported shape",
          MHD_ERR_HIP: "HIP runtime error", MHD_ERR_TOO_LONG: "pencil too long",
-         MHD_ERR_NOMEM: "out of device memory"}
+         MHD_ERR_NOMEM: "out of device memory", MHD_ERR_IO: "file error"}
 
 # every symbol include/mhd_hip.h declares, with (restype, argtypes)
 _fp = C.POINTER(C.c_float)
@@ -95,6 +96,10 @@ SIGNATURES = {
     "mhd_ensemble_get_config": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "mhd_ensemble_tune": (C.c_int, [_vp, C.c_int, _fp]),
     "mhd_ensemble_set_precision": (C.c_int, [_vp, C.c_int]),
+    # the reference's text formats
+    "mhd_read_source_init": (C.c_int, [C.c_char_p, _vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "mhd_write_source_init": (C.c_int, [C.c_char_p, _vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "mhd_write_check_dumps": (C.c_int, [C.c_char_p, _vp, _vp, C.c_int, C.c_int, C.c_int]),
     # learned corrector
     "mhd_plan_set_tf": (C.c_int, [_vp, C.c_float]),
     "mhd_plan_state_changed": (C.c_int, [_vp]),

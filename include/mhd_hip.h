@@ -71,6 +71,7 @@ float calcfl(float *u, float *b, int *nx, int *ny, int *nz);
 #define MHD_ERR_HIP -3       /* a HIP runtime call failed */
 #define MHD_ERR_TOO_LONG -4  /* pencil longer than mhd_max_pencil() */
 #define MHD_ERR_NOMEM -5     /* device allocation failed */
+#define MHD_ERR_IO -6        /* a file could not be opened, read or written */
 
 typedef void *mhd_stream_t; /* a hipStream_t; NULL is the null stream */
 
@@ -225,6 +226,19 @@ int mhd_ensemble_get_config(mhd_ensemble *ens, int *shape, int *grid, int *block
 int mhd_ensemble_tune(mhd_ensemble *ens, int reps, float *best_ms);
 /* Selects MHD_PREC_EXACT or MHD_PREC_FAST for the ensemble's later steps. */
 int mhd_ensemble_set_precision(mhd_ensemble *ens, int mode);
+
+/* ---- the reference's text formats (host arrays, no GPU needed) ---- */
+/* replaces the IC read of C/init.c:34-52: 8 whitespace-separated floats
+ * per cell (rho, m1, m2, m3, E, b1, b2, b3), cells with i outermost and k
+ * innermost, converted as fscanf("%f") converts them. */
+int mhd_read_source_init(const char *path, float *u, float *b, int nx, int ny, int nz);
+/* The inverse, with 9 significant digits, so every float reads back as
+ * itself (no reference counterpart: the reference only reads these files). */
+int mhd_write_source_init(const char *path, const float *u, const float *b, int nx, int ny, int nz);
+/* replaces the dumps of C/check_value.c:53-69: dir/C_version_check_u1..u5.dat
+ * and _b1..b3.dat, one "%8.6e" value per line, cells with k outermost and i
+ * innermost. */
+int mhd_write_check_dumps(const char *dir, const float *u, const float *b, int nx, int ny, int nz);
 
 /* ------------------------------------------------------------------ */
 /* Part 3: learned coarse-grid corrector (no counterpart in the reference) */
