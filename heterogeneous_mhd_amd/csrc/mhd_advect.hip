@@ -26,7 +26,7 @@ int launch_cfg(float *b_out, const float *b_main, const float *uu, const float *
     constexpr int pb = BLOCK / TP;
     const long npen = (long)ny * nz;
     const long nblk = (npen + pb - 1) / pb;
-    const size_t lds = sizeof(float) * 3 * kLines * (size_t)nx * pb;
+    const size_t lds = sizeof(float) * 2 * kLines * (size_t)nx * pb;
     auto k = nx == M * TP ? advect_kernel<BLOCK, M, FAST, TP, true> : advect_kernel<BLOCK, M, FAST, TP, false>;
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

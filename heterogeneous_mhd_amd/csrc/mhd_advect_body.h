@@ -59,10 +59,9 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
     const int km = k == 0 ? nz - 1 : k - 1, kp = k + 1 == nz ? 0 : k + 1;
     const float z = 0.0f, h = 0.5f;
 
-    // three stencil arrays of kLines*nx floats per pencil, line-major
-    float *X = lds + (size_t)lp * 3 * kLines * nx;
+    // two stencil arrays of kLines*nx floats per pencil, line-major
+    float *X = lds + (size_t)lp * 2 * kLines * nx;
     float *Y = X + kLines * nx;
-    float *Z = Y + kLines * nx;
 
     // pencil (row) offsets in cells: jm, c, jp, km, kp
     const size_t row = (size_t)nx;
@@ -136,32 +135,38 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
         }
     }
     __syncthreads();
+    // ---- half step b1 = b - (flux1 - flux1(i-1))*dt/2, C/tvdb.c:48-52, with
+    // the upwind flux1 = vh > 0 ? b*vg : (b*vg)(i+1) of C/tvdb.c:44-46 formed
+    // in registers for i and i-1 (vh(i-1) by the same expression its owner
+    // uses), so no third LDS array is needed; X is rewritten after a barrier
+    float bh[M][kLines];
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
         if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1;
+            const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
-            for (int t = 0; t < kLines; t++)
-                Z[t * nx + i] = vh[r][t] > 0.0f ? X[t * nx + i] : X[t * nx + ip];  // flux1
+            for (int t = 0; t < kLines; t++) {
+                const float vhm = FAST ? 0.5f * (Y[t * nx + im] + Y[t * nx + i])
+                                       : h * Y[t * nx + im] + h * Y[t * nx + i] + z;
+                const float f1 = vh[r][t] > 0.0f ? X[t * nx + i] : X[t * nx + ip];
+                const float f1m = vhm > 0.0f ? X[t * nx + im] : X[t * nx + i];
+                if constexpr (FAST) {
+                    bh[r][t] = fmaf(-0.5f * dt, f1 - f1m, bl[r][t]);
+                } else {
+                    const float d = (f1 - f1m) * dt;
+                    bh[r][t] = (float)((double)bl[r][t] - (double)d / 2.0);
+                }
+            }
         }
     }
     __syncthreads();
-    // ---- half step b1 = b - (flux1 - flux1(i-1))*dt/2, C/tvdb.c:48-52
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
         if (live && i < nx) {
-            const int im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
-            for (int t = 0; t < kLines; t++) {
-                if constexpr (FAST) {
-                    X[t * nx + i] = fmaf(-0.5f * dt, Z[t * nx + i] - Z[t * nx + im], bl[r][t]);
-                } else {
-                    const float d = (Z[t * nx + i] - Z[t * nx + im]) * dt;
-                    X[t * nx + i] = (float)((double)bl[r][t] - (double)d / 2.0);
-                }
-            }
+            for (int t = 0; t < kLines; t++) X[t * nx + i] = bh[r][t];
         }
     }
     __syncthreads();
@@ -212,7 +217,7 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
         const int i = lt + r * tp;
         if (live && i < nx) {
 #pragma unroll
-            for (int t = 0; t < kLines; t++) Z[t * nx + i] = fx[r][t];
+            for (int t = 0; t < kLines; t++) Y[t * nx + i] = fx[r][t];  // vg is no longer read
         }
     }
     __syncthreads();
@@ -225,7 +230,7 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
             const int im = i == 0 ? nx - 1 : i - 1;
             float fm[kLines];
 #pragma unroll
-            for (int t = 0; t < kLines; t++) fm[t] = Z[t * nx + im];
+            for (int t = 0; t < kLines; t++) fm[t] = Y[t * nx + im];
             const float b2n = bl[r][0] - (fx[r][0] - fm[0]);
             const float b3n = bl[r][2] - (fx[r][2] - fm[2]);
             const float b1n = (((b1m[r] - fm[0]) + fm[1]) - fm[2]) + fm[3];

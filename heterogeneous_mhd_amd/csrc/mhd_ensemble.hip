@@ -236,7 +236,7 @@ int launch_sweep(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, floa
 {
     constexpr int pb = BLOCK / TP;
     if (kind == 1) {
-        const size_t lds = sizeof(float) * 12 * (size_t)nmax * pb;
+        const size_t lds = sizeof(float) * 2 * kLines * (size_t)nmax * pb;
         return fast ? launch_advect_k<BLOCK, M, TP, true>(E, Bo, Bm, Ui, Bi, lds, grid, s)
                     : launch_advect_k<BLOCK, M, TP, false>(E, Bo, Bm, Ui, Bi, lds, grid, s);
     }
@@ -248,7 +248,7 @@ int launch_sweep(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, floa
         return fast ? launch_fluid_k<BLOCK, M, TP, false, true>(E, Uo, Ui, Bi, lds, ws, grid, s)
                     : launch_fluid_k<BLOCK, M, TP, false, false>(E, Uo, Ui, Bi, lds, ws, grid, s);
     }
-    const size_t lds = sizeof(float) * 12 * (size_t)nmax * pb;
+    const size_t lds = sizeof(float) * 10 * (size_t)nmax * pb;  // >= the advect's 2*kLines
     if (cfl)
         return fast ? launch_fused_k<BLOCK, M, TP, true, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s)
                     : launch_fused_k<BLOCK, M, TP, true, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s);

@@ -42,7 +42,8 @@ int launch_cfg(float *u_out, float *b_out, const float *uu, const float *bu, int
     constexpr int pb = BLOCK / TP;
     const long npen = (long)ny * nz;
     const long nblk = (npen + pb - 1) / pb;
-    const size_t lds = sizeof(float) * 3 * kLines * (size_t)nx * pb;  // >= the fluid's 10*nx*pb
+    // the fluid sweep's 10 floats per cell exceed the advect's 2*kLines
+    const size_t lds = sizeof(float) * 10 * (size_t)nx * pb;
     auto k = nx == M * TP ? advect_fluid_kernel<BLOCK, M, CFL, FAST, TP, true>
                           : advect_fluid_kernel<BLOCK, M, CFL, FAST, TP, false>;
     if (lds > 65536)
