@@ -533,6 +533,7 @@ def main(argv=None):
     f.add_argument("--noise", type=float, default=1.0)
     f.add_argument("--fine", type=int, default=256)
     f.add_argument("--out", default=str(WEIGHTS))
+    f.add_argument("--rollout-alpha", type=float, default=None)
     f.add_argument("--alphas", default="0.5,1.0")
     f.add_argument("--val-seeds", type=int, default=2)
     f.add_argument("--val-steps", type=int, default=200)
@@ -592,7 +593,8 @@ def main(argv=None):
     elif a.cmd == "finetune":
         n = a.fine // 4
         net = CorrectorNet.load(a.init)
-        net.meta.pop("alpha", None)  # collect with the unrelaxed correction
+        if a.rollout_alpha is not None:  # else the stored relaxation drives the rollouts
+            net.meta["alpha"] = a.rollout_alpha
         seeds = list(range(2000, 2000 + a.seeds))
         t0 = time.perf_counter()
         pairs = make_onpolicy_pairs(net, seeds, a.rollout, a.every, fine=a.fine, skip=(0.0, a.skip),

@@ -57,3 +57,12 @@ def test_reference_style_c_driver_compiles_and_links(tmp_path):
     subprocess.run(["gcc", "-O2", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src), "-o",
                     str(exe), "-L", str(lib), "-lmhd_hip", f"-Wl,-rpath,{lib}", "-lm"], check=True)
     assert exe.exists()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; N > 1 exits with a message, before any GPU work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr

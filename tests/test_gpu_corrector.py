@@ -223,11 +223,13 @@ def test_advance_to_lands_on_the_time(gpu):
                     reason="no trained corrector weights in the tree")
 def test_trained_corrector_beats_plain_coarse_step(gpu):
     """On a held-out seed, the corrected 64^3 run is closer to the 4x
-    downsampled 256^3 run than the plain 64^3 run after 40 coarse steps."""
+    downsampled 256^3 run than the plain 64^3 run after 40 coarse steps, by
+    at least 20% (the committed weights give about 40% at 50 steps on every
+    one of 50 held-out seeds)."""
     from heterogeneous_mhd_amd import corrector
     print("trained corrector, held-out seed 5000, 40 steps", flush=True)
     net = corrector.CorrectorNet.load()
     rows = corrector.evaluate(net, [5000], 40, log=print)
     r = rows[0]
-    assert r["corrected"]["mean"] < r["plain"]["mean"]
+    assert r["corrected"]["mean"] < 0.8 * r["plain"]["mean"]
     assert r["fallbacks"]["rejected"] <= 40
