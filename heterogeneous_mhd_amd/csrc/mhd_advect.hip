@@ -15,7 +15,7 @@ __global__ __launch_bounds__(BLOCK) void advect_kernel(float *b_out, const float
     const int pb = BLOCK / tp;
     const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
     float keep[M][5];
-    advect_group<BLOCK, M, FAST, TP, FIT>(lds, b_out, b_main, uu, bu, nx, ny, nz, tp, grp,
+    advect_group<BLOCK, M, FAST, TP, FIT>((lds_f *)lds, b_out, b_main, uu, bu, nx, ny, nz, tp, grp,
                                           dtp ? *dtp : dtv, keep);
 }
 

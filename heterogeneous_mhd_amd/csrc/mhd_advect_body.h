@@ -39,7 +39,7 @@ constexpr int kLines = 4;  // A(j), A(j+1), B(k), B(k+1)
 // multiple of 64), so it is taken from the first lane and the per-pencil
 // address arithmetic runs on the scalar unit.
 template <int BLOCK, int M, bool FAST, int TPC = 0, bool FIT = false>
-__device__ __forceinline__ void advect_group(float *lds, float *b_out, const float *b_main,
+__device__ __forceinline__ void advect_group(lds_f *lds, float *b_out, const float *b_main,
                                              const float *__restrict__ uu,
                                              const float *__restrict__ bu, int nx_, int ny, int nz,
                                              int tp_, int grp, float dt, float (&keep)[M][5])
@@ -48,11 +48,13 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
     const int tp = TPC > 0 ? TPC : tp_;
     const int nx = FIT ? M * TPC : nx_;
     const int pb = BLOCK / tp;
-    const int lp = __builtin_amdgcn_readfirstlane(threadIdx.x / tp);
+    constexpr bool ONE = FIT && BLOCK == TPC;  // one pencil per workgroup, as in fluid_group
+    const int lp = ONE ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x / tp);
     const int lt = threadIdx.x - lp * tp;
+    if constexpr (FIT) __builtin_assume(lt >= 0 && lt < TPC);
     const int npen = ny * nz;
     const int pen = grp * pb + lp;
-    const bool live = pen < npen;
+    const bool live = ONE || pen < npen;
     const int pc = live ? pen : 0;             // in-range pencil for the clamped loads
     const int j = pc % ny, k = pc / ny;
     const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
@@ -60,8 +62,8 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
     const float z = 0.0f, h = 0.5f;
 
     // two stencil arrays of kLines*nx floats per pencil, line-major
-    float *X = lds + (size_t)lp * 2 * kLines * nx;
-    float *Y = X + kLines * nx;
+    lds_f *X = lds + lp * 2 * kLines * nx;
+    lds_f *Y = X + kLines * nx;
 
     // pencil (row) offsets in cells: jm, c, jp, km, kp
     const size_t row = (size_t)nx;
@@ -180,7 +182,7 @@ __device__ __forceinline__ void advect_group(float *lds, float *b_out, const flo
             const int im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
             for (int t = 0; t < kLines; t++) {
-                const float *bb = X + t * nx, *vv = Y + t * nx;
+                const lds_f *bb = X + t * nx, *vv = Y + t * nx;
                 float w;
                 if constexpr (FAST) {
                     // the upwind pair {wm, wp} is {(w1-w0)/2, (w-wa)/2} for vh > 0 and

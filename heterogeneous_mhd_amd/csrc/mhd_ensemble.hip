@@ -68,7 +68,7 @@ __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *Uout,
         const int n = E.n[box];
         __syncthreads();  // the previous item's last LDS reads are done
         // record index = item: each box's records are contiguous, in memory order
-        fluid_group<BLOCK, M, CFL, FAST, false, TP>(lds, red, Uout + E.uoff[box], Uin + E.uoff[box],
+        fluid_group<BLOCK, M, CFL, FAST, false, TP>((lds_f *)lds, (lds_f *)red, Uout + E.uoff[box], Uin + E.uoff[box],
                                                     Bf + E.boff[box], n, n, n, TP,
                                                     item - E.gpre[box], E.dt[box], ws, item);
     }
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout
         const long long bo = E.boff[box];
         __syncthreads();
         float keep[M][5];
-        advect_group<BLOCK, M, FAST, TP>(lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n,
+        advect_group<BLOCK, M, FAST, TP>((lds_f *)lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n,
                                          n, TP, item - E.gpre[box], E.dt[box], keep);
     }
 }
@@ -107,10 +107,10 @@ __global__ __launch_bounds__(BLOCK) void ens_fused_kernel(EnsDev E, float *Uout,
         const int grp = item - E.gpre[box];
         __syncthreads();  // the previous item's last LDS reads are done
         float keep[M][5];
-        advect_group<BLOCK, M, FAST, TP>(lds, Bout + bo, Bin + bo, Uin + uo, Bin + bo, n, n, n, TP,
+        advect_group<BLOCK, M, FAST, TP>((lds_f *)lds, Bout + bo, Bin + bo, Uin + uo, Bin + bo, n, n, n, TP,
                                          grp, E.dt[box], keep);
         __syncthreads();
-        fluid_group<BLOCK, M, CFL, FAST, true, TP>(lds, red, Uout + uo, Uin + uo, Bin + bo, n, n, n, TP,
+        fluid_group<BLOCK, M, CFL, FAST, true, TP>((lds_f *)lds, (lds_f *)red, Uout + uo, Uin + uo, Bin + bo, n, n, n, TP,
                                                    grp, E.dt[box], ws, item, keep);
     }
 }

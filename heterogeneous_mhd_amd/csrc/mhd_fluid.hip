@@ -15,7 +15,7 @@ __global__ __launch_bounds__(BLOCK) void fluidx_kernel(float *u_out, const float
     const int pb = BLOCK / tp;
     const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
     const float dt = dtp ? *dtp : dtv;
-    fluid_group<BLOCK, M, CFL, FAST, false, TP, FIT>(lds, red, u_out, uu, bu, nx, ny, nz, tp, grp, dt,
+    fluid_group<BLOCK, M, CFL, FAST, false, TP, FIT>((lds_f *)lds, (lds_f *)red, u_out, uu, bu, nx, ny, nz, tp, grp, dt,
                                                      co.ws, grp);
 }
 

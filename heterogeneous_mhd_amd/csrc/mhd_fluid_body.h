@@ -49,6 +49,13 @@
 namespace mhd {
 namespace {
 
+// LDS-qualified pointee types: index arithmetic on them is 32-bit, so the
+// constant part of a stencil offset folds into the ds_* immediate.
+using lds_f = __attribute__((address_space(3))) float;
+typedef float v4f __attribute__((ext_vector_type(4)));  // a builtin vector: no class operators
+using lds_v4 = __attribute__((address_space(3))) v4f;
+using lds_u = __attribute__((address_space(3))) unsigned;
+
 // mhdflux for one cell (C/mhdflux.c:23-45).  Returns the cell's freezing
 // speed candidate; v receives the un-normalised flux.
 __device__ __forceinline__ float flux_cell(const float q[5], const float bc[3], float v[5])
@@ -91,11 +98,15 @@ __device__ __forceinline__ float flux_cell_fast(const float q[5], const float bc
     return fabsf(vx) + __builtin_amdgcn_sqrtf(fabsf(fmaf(gam, p, bsq) * rr));
 }
 
-// 2ab/(a+b) where a*b > 0, FP32 with the hardware reciprocal.
+// 2ab/(a+b) where a and b have the same sign, else 0, in FP32 with the
+// hardware reciprocal, as (a|b| + |a|b) / (|a| + |b|): the numerator is 0
+// when the signs differ (abs is a free source modifier), and the denominator
+// is kept >= FLT_MIN so that a = b = 0 gives 0; there is no compare or select.
+// Where |a| + |b| < FLT_MIN the numerator underflows to at most ~1e-38.
 __device__ __forceinline__ float limiter_fast(float a, float b)
 {
-    const float p = a * b;
-    return p > 0.0f ? 2.0f * p * __builtin_amdgcn_rcpf(a + b) : 0.0f;
+    const float n = fmaf(a, fabsf(b), fabsf(a) * b);
+    return n * __builtin_amdgcn_rcpf(fmaxf(fabsf(a) + fabsf(b), 0x1p-126f));
 }
 
 // C/tvd1.c:75-79: 2ab/(a+b) in double where the float product a*b > 0.
@@ -136,7 +147,7 @@ __device__ __forceinline__ unsigned dpp_u32(unsigned v)
 // exchange.  Ends with a workgroup barrier, which the caller relies on to
 // recycle the LDS stencil arrays.
 template <int BLOCK>
-__device__ __forceinline__ float pencil_max(float xf, int tp, float *red)
+__device__ __forceinline__ float pencil_max(float xf, int tp, lds_f *red)
 {
     unsigned x = __float_as_uint(xf);
     x = max(x, dpp_u32<0xB1>(x));   // quad_perm [1,0,3,2]
@@ -147,7 +158,7 @@ __device__ __forceinline__ float pencil_max(float xf, int tp, float *red)
     x = max(x, (unsigned)__shfl_xor((int)x, 32, 64));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (tp > 64) {
-        unsigned *r = reinterpret_cast<unsigned *>(red);
+        lds_u *r = (lds_u *)red;
         if (lane == 0) r[wave] = x;
         __syncthreads();
         const int wpp = tp >> 6, first = (threadIdx.x / tp) * wpp;
@@ -164,14 +175,14 @@ __device__ __forceinline__ float pencil_max(float xf, int tp, float *red)
 // range (a clamped lane rewrites the last vector with its own, identical
 // value), so there is no branch for the compiler to sink the loads into.
 template <int BLOCK, int KA, int KB>
-__device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int na, float *b_dst,
+__device__ __forceinline__ void stage_in(lds_f *a_dst, const float *a_src, int na, lds_f *b_dst,
                                          const float *b_src, int nb, bool vec)
 {
     if (vec) {
-        const float4 *sa = reinterpret_cast<const float4 *>(a_src);
-        const float4 *sb = reinterpret_cast<const float4 *>(b_src);
+        const v4f *sa = reinterpret_cast<const v4f *>(a_src);
+        const v4f *sb = reinterpret_cast<const v4f *>(b_src);
         const int na4 = na >> 2, nb4 = nb >> 2;
-        float4 ra[KA], rb[KB > 0 ? KB : 1];
+        v4f ra[KA], rb[KB > 0 ? KB : 1];
 #pragma unroll
         for (int k = 0; k < KA; k++) {
             const int f = threadIdx.x + k * BLOCK;
@@ -185,12 +196,12 @@ __device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int n
 #pragma unroll
         for (int k = 0; k < KA; k++) {
             const int f = threadIdx.x + k * BLOCK;
-            reinterpret_cast<float4 *>(a_dst)[f < na4 ? f : na4 - 1] = ra[k];
+            ((lds_v4 *)a_dst)[f < na4 ? f : na4 - 1] = ra[k];
         }
 #pragma unroll
         for (int k = 0; k < KB; k++) {
             const int f = threadIdx.x + k * BLOCK;
-            reinterpret_cast<float4 *>(b_dst)[f < nb4 ? f : nb4 - 1] = rb[k];
+            ((lds_v4 *)b_dst)[f < nb4 ? f : nb4 - 1] = rb[k];
         }
     } else {
         for (int f = threadIdx.x; f < na; f += BLOCK) a_dst[f] = a_src[f];
@@ -203,7 +214,7 @@ __device__ __forceinline__ void stage_in(float *a_dst, const float *a_src, int n
 // caller has synchronised the workgroup since its last LDS read.
 // TPC / FIT: as for advect_group (compile-time threads per pencil and nx).
 template <int BLOCK, int M, bool CFL, bool FAST, bool FUSED = false, int TPC = 0, bool FIT = false>
-__device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out, const float *uu,
+__device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out, const float *uu,
                                             const float *__restrict__ bu, int nx_, int ny, int nz,
                                             int tp_, int grp, float dt, void *cfl_ws, int rec,
                                             const float (*keep)[5] = nullptr)
@@ -213,19 +224,23 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
     const int nx = FIT ? M * TPC : nx_;
     const int pb = BLOCK / tp;                 // pencils per workgroup
     // this thread's pencil in the group: uniform over the wavefront
-    const int lp = __builtin_amdgcn_readfirstlane(threadIdx.x / tp);
+    // ONE: one pencil per workgroup and one workgroup per pencil (the
+    // launchers' grid), so every thread's cells exist and no test is needed
+    constexpr bool ONE = FIT && BLOCK == TPC;
+    const int lp = ONE ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x / tp);
     const int lt = threadIdx.x - lp * tp;      // this thread's rank in its pencil
+    if constexpr (FIT) __builtin_assume(lt >= 0 && lt < TPC);
     const int npen = ny * nz;
     const int pen0 = grp * pb;
     const int npb = npen - pen0 < pb ? npen - pen0 : pb;  // pencils present
     const int pen = pen0 + lp;
-    const bool live = lp < npb;
+    const bool live = ONE || lp < npb;
     const int j = pen % ny, k = pen / ny;
     const int jp = j + 1 == ny ? 0 : j + 1, kp = k + 1 == nz ? 0 : k + 1;  // C/fluidx.c:52-53
 
-    float *A = lds;                            // 5*nx*pb floats
-    float *B = lds + 5 * nx * pb;              // 5*nx*pb floats
-    float *Ap = A + 5 * nx * lp, *Bp = B + 5 * nx * lp;
+    lds_f *A = lds;                            // 5*nx*pb floats
+    lds_f *B = lds + 5 * nx * pb;              // 5*nx*pb floats
+    lds_f *Ap = A + 5 * nx * lp, *Bp = B + 5 * nx * lp;
 
     const float z = 0.0f, h = 0.5f;
     constexpr int KU = (5 * M + 3) / 4, KB = (3 * M + 3) / 4;  // 16-byte loads per thread
@@ -265,7 +280,7 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
 
     float q[M][5], bc[M][3], v[M][5], bf[CFL ? M : 1][3];
     float cmax = 0.0f;
-    const float *Bs = B + 3 * nx * lp;         // b staging of this pencil (3 floats/cell)
+    const lds_f *Bs = B + 3 * nx * lp;         // b staging of this pencil (3 floats/cell)
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
@@ -294,6 +309,7 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
         const bool norm = c1 > 0.0f;
         const bool fast = c1 >= 0x1p-30f && c1 <= 0x1p30f;
         const float rc = norm ? 1.0f / c1 : 0.0f;
+        const float g = norm ? 1.0f : 0.0f;
 #pragma unroll
         for (int r = 0; r < M; r++) {
             const int i = lt + r * tp;
@@ -301,14 +317,11 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
 #pragma unroll
                 for (int c = 0; c < 5; c++) {
                     if constexpr (FAST) {
-                        // norm is uniform over the pencil: a branch, not a select
-                        if (norm) {
-                            s[r][c] = fmaf(c1, q[r][c], v[r][c]);
-                            Ap[5 * i + c] = fmaf(c1, q[r][c], -v[r][c]);
-                        } else {
-                            s[r][c] = 0.0f;
-                            Ap[5 * i + c] = 0.0f;
-                        }
+                        // c*(u +- v/c) = c*u +- v; a pencil with c = 0 has zero
+                        // fluxes, which the pencil-uniform factor g gives
+                        const float vg = v[r][c] * g;
+                        s[r][c] = fmaf(c1, q[r][c], vg);
+                        Ap[5 * i + c] = fmaf(c1, q[r][c], -vg);
                     } else {
                         const float vv = norm ? div_c(v[r][c], c1, rc, fast) : v[r][c];
                         const float wr = q[r][c] + vv + z;
@@ -360,6 +373,7 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
         const bool norm = c2 > 0.0f;
         const bool fast = c2 >= 0x1p-30f && c2 <= 0x1p30f;
         const float rc = norm ? 1.0f / c2 : 0.0f;
+        const float g = norm ? 1.0f : 0.0f;
 #pragma unroll
         for (int r = 0; r < M; r++) {
             const int i = lt + r * tp;
@@ -367,14 +381,9 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
 #pragma unroll
                 for (int c = 0; c < 5; c++) {
                     if constexpr (FAST) {
-                        const float u1 = s[r][c];
-                        if (norm) {
-                            s[r][c] = fmaf(c2, u1, v[r][c]);
-                            Bp[5 * i + c] = fmaf(c2, u1, -v[r][c]);
-                        } else {
-                            s[r][c] = 0.0f;
-                            Bp[5 * i + c] = 0.0f;
-                        }
+                        const float u1 = s[r][c], vg = v[r][c] * g;
+                        s[r][c] = fmaf(c2, u1, vg);
+                        Bp[5 * i + c] = fmaf(c2, u1, -vg);
                         Ap[5 * i + c] = s[r][c];
                     } else {
                         const float vv = norm ? div_c(v[r][c], c2, rc, fast) : v[r][c];
@@ -487,8 +496,8 @@ __device__ __forceinline__ void fluid_group(float *lds, float *red, float *u_out
     {
         float *go = u_out + (size_t)pen0 * nx * 5;
         if (vec) {
-            const float4 *s4 = reinterpret_cast<const float4 *>(B);
-            float4 *d4 = reinterpret_cast<float4 *>(go);
+            const lds_v4 *s4 = (const lds_v4 *)B;
+            v4f *d4 = reinterpret_cast<v4f *>(go);
             const int n4 = nu >> 2;
 #pragma unroll
             for (int kk = 0; kk < KU; kk++) {

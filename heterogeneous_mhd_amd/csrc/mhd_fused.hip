@@ -29,9 +29,9 @@ __global__ __launch_bounds__(BLOCK) void advect_fluid_kernel(float *u_out, float
     const int grp = sweep_group(blockIdx.x, gridDim.x, ny % pb == 0 ? ny / pb : 0, nz);
     const float dt = dtp ? *dtp : dtv;
     float keep[M][5];
-    advect_group<BLOCK, M, FAST, TP, FIT>(lds, b_out, bu, uu, bu, nx, ny, nz, tp, grp, dt, keep);
+    advect_group<BLOCK, M, FAST, TP, FIT>((lds_f *)lds, b_out, bu, uu, bu, nx, ny, nz, tp, grp, dt, keep);
     __syncthreads();  // the advect's last LDS reads are done: the fluid sweep reuses LDS
-    fluid_group<BLOCK, M, CFL, FAST, true, TP, FIT>(lds, red, u_out, uu, bu, nx, ny, nz, tp, grp, dt,
+    fluid_group<BLOCK, M, CFL, FAST, true, TP, FIT>((lds_f *)lds, (lds_f *)red, u_out, uu, bu, nx, ny, nz, tp, grp, dt,
                                                     co.ws, grp, keep);
 }
 

@@ -29,17 +29,30 @@ constexpr int kMaxPencil = 2048;
 // consecutive pencils along j (one contiguous run of cells); there are
 // G = gj * nz groups with gj = ny / pb.  Workgroups are dispatched round-robin
 // over the 8 XCDs of an MI355X, each with its own L2, so (a) every XCD gets a
-// contiguous share of the schedule, and (b) the schedule walks 8 x 8 tiles of
-// groups in (j, k): the neighbour pencils j+-1 and k+-1 a sweep reads are then
-// read by the same XCD at about the same time and hit its L2.  Shapes that do
-// not divide evenly keep the plain order.  Returns the group index in memory
-// order.
+// contiguous share of the schedule, and (b) that share is walked so that the
+// neighbour pencils j+-1 and k+-1 a sweep reads were read by the same XCD a
+// short while before and still sit in its L2:
+//  * strips: kStripW groups wide in j, walked down all of k (j fastest), when
+//    gj is a multiple of 8 * kStripW.  Row k+-1 of a strip is kStripW groups
+//    away; only the two strip edges read from outside.
+//  * else 8 x 8 tiles of groups in (j, k) when both divide;
+//  * else the plain order.
+// Returns the group index in memory order.
+#ifndef MHD_STRIP_W
+#define MHD_STRIP_W 32
+#endif
+constexpr int kStripW = MHD_STRIP_W;
 __device__ __forceinline__ int sweep_group(int bid, int G, int gj, int nz)
 {
     constexpr int kXcd = 8, kT = 8;
     int s = bid;
     if (G % kXcd == 0) s = (bid % kXcd) * (G / kXcd) + bid / kXcd;
-    if (gj % kT == 0 && nz % kT == 0 && G == gj * nz) {
+    if (G != gj * nz) return s;
+    if (kStripW > 0 && gj % (kXcd * kStripW) == 0) {
+        const int strip = s / (kStripW * nz), w = s % (kStripW * nz);
+        return (w / kStripW) * gj + strip * kStripW + w % kStripW;
+    }
+    if (gj % kT == 0 && nz % kT == 0) {
         const int tile = s / (kT * kT), w = s % (kT * kT);
         const int tiles_j = gj / kT;
         const int tj = tile % tiles_j, tk = tile / tiles_j;
