@@ -35,11 +35,12 @@ import numpy as np
 PEAK_HBM_GBS = 8000.0
 # compulsory bytes per cell per call at each operator's C-call boundary (FP32)
 ALGO_BYTES = {"fluidx": 52, "advectbyzx": 32, "transpose12": 64, "transpose13": 64, "calcfl": 32,
-              "advect_fluidx": 64}
+              "advect_fluidx": 64, "advect2_fluidx": 64}
 # launches per step of the plan: the second half's three advectbyzx+fluidx
-# pairs run as one fused kernel, and calcfl is fused into the last of them
-CALLS_PER_STEP = {"fluidx": 3, "advectbyzx": 3, "transpose12": 2, "transpose13": 2,
-                  "advect_fluidx": 3}
+# pairs run fused, the first of them together with the first half's last
+# advectbyzx (advect2_fluidx), and calcfl is fused into the last pair
+CALLS_PER_STEP = {"fluidx": 3, "advectbyzx": 2, "transpose12": 2, "transpose13": 2,
+                  "advect_fluidx": 2, "advect2_fluidx": 1}
 # the reference decomposition the step roofline is priced on (792 B/cell)
 REF_CALLS = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13": 2, "calcfl": 1}
 METRIC = "cell-updates/s & % HBM roofline per 3D step"
@@ -57,6 +58,7 @@ def kernel_times(box, reps: int, precision: str = "exact"):
         "fluidx": lambda: ops.fluidx(u2, b, nx, ny, nz, dt, u, b, precision=precision),
         "advectbyzx": lambda: ops.advectbyzx(u, b2, nx, ny, nz, dt, u, b, precision=precision),
         "advect_fluidx": lambda: ops.advect_fluidx(u2, b2, nx, ny, nz, dt, u, b, precision=precision),
+        "advect2_fluidx": lambda: ops.advect2_fluidx(u2, b2, nx, ny, nz, dt, u, b, precision=precision),
         "transpose12": lambda: ops.transpose12(u2, b2, u, b, nx, ny, nz),
         "transpose13": lambda: ops.transpose13(u2, b2, u, b, nx, ny, nz),
         "calcfl": lambda: ops.calcfl_device(u, b, nx, ny, nz),

@@ -38,7 +38,15 @@ constexpr int kLines = 4;  // A(j), A(j+1), B(k), B(k+1)
 // bounds test.  The pencil index is uniform over a wavefront (tp is a
 // multiple of 64), so it is taken from the first lane and the per-pencil
 // address arithmetic runs on the scalar unit.
-template <int BLOCK, int M, bool FAST, int TPC = 0, bool FIT = false>
+//
+// NPASS = 2: two advectbyzx calls with the same u (C/mhd.c:93 and :95, the
+// end of the first half and the start of the second).  The face velocities
+// depend on u only, so they are computed once; the second pass takes its
+// fields from the first pass's keep (the new b of this pencil, b2 of (j+1,k)
+// and b3 of (j,k+1): every value the second call reads), so the
+// intermediate field never goes to HBM.  Only the second pass writes b_out;
+// the results are bitwise those of the two calls.
+template <int BLOCK, int M, bool FAST, int TPC = 0, bool FIT = false, int NPASS = 1>
 __device__ __forceinline__ void advect_group(lds_f *lds, float *b_out, const float *b_main,
                                              const float *__restrict__ uu,
                                              const float *__restrict__ bu, int nx_, int ny, int nz,
@@ -137,114 +145,140 @@ __device__ __forceinline__ void advect_group(lds_f *lds, float *b_out, const flo
         }
     }
     __syncthreads();
-    // ---- half step b1 = b - (flux1 - flux1(i-1))*dt/2, C/tvdb.c:48-52, with
-    // the upwind flux1 = vh > 0 ? b*vg : (b*vg)(i+1) of C/tvdb.c:44-46 formed
-    // in registers for i and i-1 (vh(i-1) by the same expression its owner
-    // uses), so no third LDS array is needed; X is rewritten after a barrier
-    float bh[M][kLines];
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+    for (int pass = 0; pass < NPASS; pass++) {
+        if (pass > 0) {
+            // the second call: fields from the first one's results, same vg
+            __syncthreads();  // the first pass's last reads of Y are done
 #pragma unroll
-            for (int t = 0; t < kLines; t++) {
-                const float vhm = FAST ? 0.5f * (Y[t * nx + im] + Y[t * nx + i])
-                                       : h * Y[t * nx + im] + h * Y[t * nx + i] + z;
-                const float f1 = vh[r][t] > 0.0f ? X[t * nx + i] : X[t * nx + ip];
-                const float f1m = vhm > 0.0f ? X[t * nx + im] : X[t * nx + i];
-                if constexpr (FAST) {
-                    bh[r][t] = fmaf(-0.5f * dt, f1 - f1m, bl[r][t]);
-                } else {
-                    const float d = (f1 - f1m) * dt;
-                    bh[r][t] = (float)((double)bl[r][t] - (double)d / 2.0);
-                }
-            }
-        }
-    }
-    __syncthreads();
+            for (int r = 0; r < M; r++) {
+                const int i = lt + r * tp;
+                if (live && i < nx) {
+                    bl[r][0] = keep[r][1];  // b2 of this pencil
+                    bl[r][1] = keep[r][3];  // b2 of (j+1, k)
+                    bl[r][2] = keep[r][2];  // b3 of this pencil
+                    bl[r][3] = keep[r][4];  // b3 of (j, k+1)
+                    b1m[r] = keep[r][0];
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-#pragma unroll
-            for (int t = 0; t < kLines; t++) X[t * nx + i] = bh[r][t];
-        }
-    }
-    __syncthreads();
-    // ---- limited flux, C/tvdb.c:54-78
-    float fx[M][kLines];
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1, ipp = ip + 1 == nx ? 0 : ip + 1;
-            const int im = i == 0 ? nx - 1 : i - 1;
-#pragma unroll
-            for (int t = 0; t < kLines; t++) {
-                const lds_f *bb = X + t * nx, *vv = Y + t * nx;
-                float w;
-                if constexpr (FAST) {
-                    // the upwind pair {wm, wp} is {(w1-w0)/2, (w-wa)/2} for vh > 0 and
-                    // {(w0-w1)/2, (w-wa)/2} otherwise; the limiter is symmetric and
-                    // homogeneous, so dw = limiter(+-(w1-w0), w-wa)/2
-                    const bool up = vh[r][t] > 0.0f;
-                    const float w0 = vv[i] * bb[i], w1 = vv[ip] * bb[ip];
-                    const int ia = up ? im : ipp;
-                    const float wa = vv[ia] * bb[ia];
-                    w = up ? w0 : w1;
-                    const float d1 = w1 - w0;
-                    fx[r][t] = fmaf(0.5f, limiter_fast(up ? d1 : -d1, w - wa), w) * dt;
-                } else {
-                    float wp, wm;
-                    if (vh[r][t] > 0.0f) {
-                        w = vv[i] * bb[i];
-                        wp = (float)((double)(vv[ip] * bb[ip] - w) / 2.0);
-                        wm = (float)((double)(w - vv[im] * bb[im]) / 2.0);
-                    } else {
-                        w = vv[ip] * bb[ip];
-                        wp = (float)((double)(w - vv[ipp] * bb[ipp]) / 2.0);
-                        wm = (float)((double)(vv[i] * bb[i] - w) / 2.0);
+                    for (int t = 0; t < kLines; t++) {
+                        Y[t * nx + i] = vg[r][t];
+                        X[t * nx + i] = bl[r][t] * vg[r][t];
                     }
-                    float dw = 0.0f;
-                    if (wm * wp > 0.0f) dw = (float)(2.0 * (double)wm * (double)wp / (double)(wm + wp));
-                    fx[r][t] = (w + dw) * dt;
+                }
+            }
+            __syncthreads();
+        }
+        // ---- half step b1 = b - (flux1 - flux1(i-1))*dt/2, C/tvdb.c:48-52, with
+        // the upwind flux1 = vh > 0 ? b*vg : (b*vg)(i+1) of C/tvdb.c:44-46 formed
+        // in registers for i and i-1 (vh(i-1) by the same expression its owner
+        // uses), so no third LDS array is needed; X is rewritten after a barrier
+        float bh[M][kLines];
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
+                const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+#pragma unroll
+                for (int t = 0; t < kLines; t++) {
+                    const float vhm = FAST ? 0.5f * (Y[t * nx + im] + Y[t * nx + i])
+                                           : h * Y[t * nx + im] + h * Y[t * nx + i] + z;
+                    const float f1 = vh[r][t] > 0.0f ? X[t * nx + i] : X[t * nx + ip];
+                    const float f1m = vhm > 0.0f ? X[t * nx + im] : X[t * nx + i];
+                    if constexpr (FAST) {
+                        bh[r][t] = fmaf(-0.5f * dt, f1 - f1m, bl[r][t]);
+                    } else {
+                        const float d = (f1 - f1m) * dt;
+                        bh[r][t] = (float)((double)bl[r][t] - (double)d / 2.0);
+                    }
                 }
             }
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
 #pragma unroll
-            for (int t = 0; t < kLines; t++) Y[t * nx + i] = fx[r][t];  // vg is no longer read
+                for (int t = 0; t < kLines; t++) X[t * nx + i] = bh[r][t];
+            }
         }
-    }
-    __syncthreads();
-    // ---- b -= flux - flux(i-1) (C/tvdb.c:80-88); b1 gets the four edge
-    // fluxes F(i-1) in the order A1, A2, B1, B2 apply them.
+        __syncthreads();
+        // ---- limited flux, C/tvdb.c:54-78
+        float fx[M][kLines];
 #pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int im = i == 0 ? nx - 1 : i - 1;
-            float fm[kLines];
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
+                const int ip = i + 1 == nx ? 0 : i + 1, ipp = ip + 1 == nx ? 0 : ip + 1;
+                const int im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
-            for (int t = 0; t < kLines; t++) fm[t] = Y[t * nx + im];
-            const float b2n = bl[r][0] - (fx[r][0] - fm[0]);
-            const float b3n = bl[r][2] - (fx[r][2] - fm[2]);
-            const float b1n = (((b1m[r] - fm[0]) + fm[1]) - fm[2]) + fm[3];
-            float *o = b_out + (op[1] + i) * 3;
-            o[0] = b1n;
-            o[1] = b2n;
-            o[2] = b3n;
-            keep[r][0] = b1n;
-            keep[r][1] = b2n;
-            keep[r][2] = b3n;
-            keep[r][3] = bl[r][1] - (fx[r][1] - fm[1]);  // b2 of pencil (j+1, k)
-            keep[r][4] = bl[r][3] - (fx[r][3] - fm[3]);  // b3 of pencil (j, k+1)
+                for (int t = 0; t < kLines; t++) {
+                    const lds_f *bb = X + t * nx, *vv = Y + t * nx;
+                    float w;
+                    if constexpr (FAST) {
+                        // the upwind pair {wm, wp} is {(w1-w0)/2, (w-wa)/2} for vh > 0 and
+                        // {(w0-w1)/2, (w-wa)/2} otherwise; the limiter is symmetric and
+                        // homogeneous, so dw = limiter(+-(w1-w0), w-wa)/2
+                        const bool up = vh[r][t] > 0.0f;
+                        const float w0 = vv[i] * bb[i], w1 = vv[ip] * bb[ip];
+                        const int ia = up ? im : ipp;
+                        const float wa = vv[ia] * bb[ia];
+                        w = up ? w0 : w1;
+                        const float d1 = w1 - w0;
+                        fx[r][t] = fmaf(0.5f, limiter_fast(up ? d1 : -d1, w - wa), w) * dt;
+                    } else {
+                        float wp, wm;
+                        if (vh[r][t] > 0.0f) {
+                            w = vv[i] * bb[i];
+                            wp = (float)((double)(vv[ip] * bb[ip] - w) / 2.0);
+                            wm = (float)((double)(w - vv[im] * bb[im]) / 2.0);
+                        } else {
+                            w = vv[ip] * bb[ip];
+                            wp = (float)((double)(w - vv[ipp] * bb[ipp]) / 2.0);
+                            wm = (float)((double)(vv[i] * bb[i] - w) / 2.0);
+                        }
+                        float dw = 0.0f;
+                        if (wm * wp > 0.0f) dw = (float)(2.0 * (double)wm * (double)wp / (double)(wm + wp));
+                        fx[r][t] = (w + dw) * dt;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
+#pragma unroll
+                for (int t = 0; t < kLines; t++) Y[t * nx + i] = fx[r][t];  // vg is restaged from registers for a second pass
+            }
+        }
+        __syncthreads();
+        // ---- b -= flux - flux(i-1) (C/tvdb.c:80-88); b1 gets the four edge
+        // fluxes F(i-1) in the order A1, A2, B1, B2 apply them.
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx) {
+                const int im = i == 0 ? nx - 1 : i - 1;
+                float fm[kLines];
+#pragma unroll
+                for (int t = 0; t < kLines; t++) fm[t] = Y[t * nx + im];
+                const float b2n = bl[r][0] - (fx[r][0] - fm[0]);
+                const float b3n = bl[r][2] - (fx[r][2] - fm[2]);
+                const float b1n = (((b1m[r] - fm[0]) + fm[1]) - fm[2]) + fm[3];
+                if (pass == NPASS - 1) {
+                    float *o = b_out + (op[1] + i) * 3;
+                    o[0] = b1n;
+                    o[1] = b2n;
+                    o[2] = b3n;
+                }
+                keep[r][0] = b1n;
+                keep[r][1] = b2n;
+                keep[r][2] = b3n;
+                keep[r][3] = bl[r][1] - (fx[r][1] - fm[1]);  // b2 of pencil (j+1, k)
+                keep[r][4] = bl[r][3] - (fx[r][3] - fm[3]);  // b3 of pencil (j, k+1)
+            }
         }
     }
 }

@@ -293,6 +293,16 @@ int mhd_advect_fluidx_async(float *u, float *b, const float *update_u, const flo
                                     precision == MHD_PREC_FAST);
 }
 
+int mhd_advect2_fluidx_async(float *u, float *b, const float *update_u, const float *update_b,
+                             int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                             mhd_stream_t stream)
+{
+    if (precision != MHD_PREC_EXACT && precision != MHD_PREC_FAST) return MHD_ERR_ARG;
+    return mhd::launch_advect_fluid(u, b, update_u, update_b, nx, ny, nz, dt_dev, dt,
+                                    (hipStream_t)stream, nullptr, nullptr, nullptr, 0.0,
+                                    precision == MHD_PREC_FAST, 2);
+}
+
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream)
 {
@@ -339,6 +349,7 @@ struct mhd_plan {
     hipStream_t s;
     float *scal;      // [0]=t, [1]=dt, [2]=1/c, [3]=next step's unclamped dt
     void *ws;         // calcfl workspace
+    float *b3;        // a third face-field buffer (3 floats/cell), see enqueue_step
     hipGraph_t graph;
     hipGraphExec_t exec;
     const float *gptr[4];
@@ -358,9 +369,11 @@ int mhd_plan_create(mhd_plan **plan, int nx, int ny, int nz, double cfl, float t
     mhd_plan *p = new mhd_plan();
     p->nx = nx, p->ny = ny, p->nz = nz, p->cfl = cfl, p->tf = tf, p->s = (hipStream_t)stream;
     const size_t wsb = mhd::cfl_workspace_bytes(nx, ny, nz);
-    if (hipMalloc(&p->scal, 64) != hipSuccess || hipMalloc(&p->ws, wsb) != hipSuccess) {
+    const size_t b3b = sizeof(float) * 3 * (size_t)nx * ny * nz;
+    if (hipMalloc(&p->scal, 64) != hipSuccess || hipMalloc(&p->ws, wsb) != hipSuccess ||
+        hipMalloc(&p->b3, b3b) != hipSuccess) {
         note(hipGetLastError(), "plan alloc");
-        delete p;
+        mhd_plan_destroy(p);
         return MHD_ERR_NOMEM;
     }
     hipError_t e = hipMemset(p->scal, 0, 64);
@@ -383,6 +396,7 @@ void mhd_plan_destroy(mhd_plan *p)
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->scal) (void)hipFree(p->scal);
     if (p->ws) (void)hipFree(p->ws);
+    if (p->b3) (void)hipFree(p->b3);
     delete p;
 }
 
@@ -432,18 +446,24 @@ int mhd_plan_set_precision(mhd_plan *p, int mode)
 }
 
 // One step, C/mhd.c:59-124.  Operators map state -> state; the reference's
-// copy_matrix calls disappear into a ping-pong between (U0,B0) and (U1,B1),
-// and the three advectbyzx + fluidx pairs of the second half run fused.
-// The y and z fluid sweeps of the first half work in place; every other
-// operator writes the other buffer.  u changes buffer 8 times and b 10 times
-// per step, so the state ends where it started.  calcfl (:62) of the next step is fused into this
-// step's last fluid sweep, which writes 1/c and the unclamped dt to scal[2..3];
-// the step begins by applying the clamp and advancing t (:63-64).
+// copy_matrix calls disappear into a rotation over (U0,B0), (U1,B1) and a
+// third face-field buffer B2 of the plan.  The three advectbyzx + fluidx
+// pairs of the second half run fused, and the middle of the step -- the
+// first half's last advectbyzx and the second half's first pair, whose two
+// advectbyzx calls use the same u -- is one kernel whose intermediate field
+// never leaves the registers.  The y and z fluid sweeps of the first half
+// work in place; every other operator writes another buffer than it reads.
+// u changes buffer 8 times and b 9 times per step; with three b buffers the
+// state still ends where it started.  calcfl (:62) of the next step is fused
+// into this step's last fluid sweep, which writes 1/c and the unclamped dt
+// to scal[2..3]; the step begins by applying the clamp and advancing t
+// (:63-64).
 static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
 {
     const int nx = p->nx, ny = p->ny, nz = p->nz;
     const hipStream_t s = p->s;
     float *t = p->scal, *dt = p->scal + 1, *inv = p->scal + 2, *dtn = p->scal + 3;
+    float *B2 = p->b3;
     const bool f = p->fast;
     int rc = MHD_OK;
 #define RUN(x)                    \
@@ -461,17 +481,16 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
     RUN(mhd::launch_transpose13(U1, B0, U0, B1, ny, nx, nz, s));                         // :88
     // z sweeps, dims (nz, nx, ny)
     RUN(mhd::launch_fluidx(U1, U1, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :91
-    RUN(mhd::launch_advectbyzx(B1, B0, U1, B0, nz, nx, ny, dt, 0.0f, s, f));               // :93
-    // second half: each advectbyzx + fluidx pair is one fused kernel
-    RUN(mhd::launch_advect_fluid(U0, B0, U1, B1, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr,
-                                 0.0, f));                                               // :95-97
-    RUN(mhd::launch_transpose13(U1, B1, U0, B0, nz, nx, ny, s));                         // :102
+    // :93 advectbyzx and the second half's first pair (:95-97) in one kernel
+    RUN(mhd::launch_advect_fluid(U0, B1, U1, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr,
+                                 0.0, f, 2));                                            // :93-97
+    RUN(mhd::launch_transpose13(U1, B2, U0, B1, nz, nx, ny, s));                         // :102
     // y again, dims (ny, nx, nz)
-    RUN(mhd::launch_advect_fluid(U0, B0, U1, B1, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr,
+    RUN(mhd::launch_advect_fluid(U0, B1, U1, B2, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr,
                                  0.0, f));                                               // :105-107
-    RUN(mhd::launch_transpose12(U1, B1, U0, B0, ny, nx, nz, s));                         // :112
+    RUN(mhd::launch_transpose12(U1, B2, U0, B1, ny, nx, nz, s));                         // :112
     // x again; its sweep also reduces calcfl of the new state for the next step
-    RUN(mhd::launch_advect_fluid(U0, B0, U1, B1, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl,
+    RUN(mhd::launch_advect_fluid(U0, B0, U1, B2, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl,
                                  f));                                                    // :115-117
 #undef RUN
     return rc;

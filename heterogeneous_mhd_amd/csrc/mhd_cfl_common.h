@@ -66,6 +66,21 @@ __device__ __forceinline__ float cfl_cell(const float *q, const float *bb)
     return (float)((double)v + sqrt(fabs(((double)bsq * 2.0 + (double)(gam * p)) / (double)rho)));
 }
 
+// cfl_cell in FP32 only (the tolerance mode's fused CFL): hardware
+// reciprocal and square root, FMA; max_i |m_i/rho| as max_i |m_i| * |1/rho|.
+// NaN inputs still give NaN (through msq, rho or the energy term).
+__device__ __forceinline__ float cfl_cell_fast(const float *q, const float *bb)
+{
+    const float gam = 5.0f / 3.0f;
+    const float rr = __builtin_amdgcn_rcpf(q[0]);
+    const float v = fmaxf(fmaxf(fabsf(q[1]), fabsf(q[2])), fabsf(q[3])) * fabsf(rr);
+    const float msq = fmaf(q[1], q[1], fmaf(q[2], q[2], q[3] * q[3]));
+    const float bsq = fmaf(bb[0], bb[0], fmaf(bb[1], bb[1], bb[2] * bb[2]));
+    const float ps = fmaf(fmaf(-0.5f * msq, rr, q[4]), gam - 1.0f, (1.0f - 0.5f * gam) * bsq);
+    const float p = fmaf(-0.5f, bsq, ps);
+    return v + __builtin_amdgcn_sqrtf(fabsf(fmaf(2.0f, bsq, gam * p) * rr));
+}
+
 template <int BLOCK>
 __device__ __forceinline__ int cfl_block_max_int(int x, int *sh)
 {

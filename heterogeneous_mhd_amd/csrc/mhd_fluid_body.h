@@ -42,19 +42,14 @@
 //
 // With a CFL workspace the kernel also computes calcfl (C/calcfl.c:13-63) of
 // the state it writes, so the last sweep of a step yields the next step's dt
-// without another pass over HBM.
+// without another pass over HBM (in FAST, in FP32 with the hardware
+// reciprocal and square root: dt within a few ulp of the exact one).
 #pragma once
 #include "mhd_cfl_common.h"
 
 namespace mhd {
 namespace {
 
-// LDS-qualified pointee types: index arithmetic on them is 32-bit, so the
-// constant part of a stencil offset folds into the ds_* immediate.
-using lds_f = __attribute__((address_space(3))) float;
-typedef float v4f __attribute__((ext_vector_type(4)));  // a builtin vector: no class operators
-using lds_v4 = __attribute__((address_space(3))) v4f;
-using lds_u = __attribute__((address_space(3))) unsigned;
 
 // mhdflux for one cell (C/mhdflux.c:23-45).  Returns the cell's freezing
 // speed candidate; v receives the un-normalised flux.
@@ -487,7 +482,7 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
                 Bp[5 * i + c] = un[c];
             }
             if (CFL) {
-                tcf[r] = cfl_cell(un, bf[r]);
+                tcf[r] = FAST ? cfl_cell_fast(un, bf[r]) : cfl_cell(un, bf[r]);
                 if (tcf[r] != tcf[r]) lnan = toff[r] > lnan ? toff[r] : lnan;
             }
         }

@@ -22,6 +22,14 @@
 
 namespace mhd {
 
+// LDS-qualified pointee types for device code: index arithmetic on them is
+// 32-bit, so the constant part of an offset folds into the ds_* immediate.
+// v4f is a builtin 16-byte vector (no class operators, so it can live in LDS).
+using lds_f = __attribute__((address_space(3))) float;
+typedef float v4f __attribute__((ext_vector_type(4)));
+using lds_v4 = __attribute__((address_space(3))) v4f;
+using lds_u = __attribute__((address_space(3))) unsigned;
+
 // Largest pencil (first-axis length) the sweep kernels accept.
 constexpr int kMaxPencil = 2048;
 
@@ -97,10 +105,11 @@ int launch_advectbyzx(float *b_out, const float *b_main, const float *uu, const 
 
 // advectbyzx (b_out from bu, b_main = bu) followed by fluidx (u_out from uu
 // and the new b), bitwise equal to the two launches; u_out must not alias uu
-// nor b_out bu.  cfl_ws etc. as for launch_fluidx.
+// nor b_out bu.  cfl_ws etc. as for launch_fluidx.  npass = 2: advectbyzx
+// twice with the same uu, then fluidx (no CFL).
 int launch_advect_fluid(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                         int nz, const float *dt_dev, float dt_val, hipStream_t s, void *cfl_ws,
-                        float *inv_c, float *dt_next, double cfl, bool fast);
+                        float *inv_c, float *dt_next, double cfl, bool fast, int npass = 1);
 
 int launch_transpose12(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                        int nz, hipStream_t s);

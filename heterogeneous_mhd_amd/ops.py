@@ -78,6 +78,20 @@ def advect_fluidx(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
                                         _stream()), "advect_fluidx")
 
 
+def advect2_fluidx(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
+    """advectbyzx twice with the same update_u, then fluidx, as one kernel
+    (bitwise equal to the three calls; the intermediate field is not
+    written): b <- CT update of the CT update of update_b, u <- fluid sweep
+    of update_u with that b.  u, b must not alias update_u, update_b."""
+    n = nx * ny * nz
+    dp, dv = _dt(dt)
+    check(lib().mhd_advect2_fluidx_async(_dev(u, 5 * n, "u"), _dev(b, 3 * n, "b"),
+                                         _dev(update_u, 5 * n, "update_u"),
+                                         _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv,
+                                         MHD_PREC_FAST if precision == "fast" else MHD_PREC_EXACT,
+                                         _stream()), "advect2_fluidx")
+
+
 def transpose12(u, b, update_u, update_b, nx, ny, nz, iter=0):
     """x<->y swap of update_* into u, b (C/transpose12.c:7-47)."""
     del iter

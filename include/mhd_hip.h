@@ -114,6 +114,15 @@ int mhd_advectbyzx_fast_async(float *b, const float *b_main, const float *update
 int mhd_advect_fluidx_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                             int ny, int nz, const float *dt_dev, float dt, int precision,
                             mhd_stream_t stream);
+/* advectbyzx twice with the same update_u (b from update_b, then from that
+ * result), then fluidx (u from update_u and the final b), in one kernel,
+ * bitwise equal to the three calls -- the middle of the step, C/mhd.c:93
+ * and :95-97 (no reference counterpart as one call).  The intermediate field
+ * is not written.  u must not alias update_u, b not update_b.
+ * precision: MHD_PREC_EXACT or MHD_PREC_FAST. */
+int mhd_advect2_fluidx_async(float *u, float *b, const float *update_u, const float *update_b,
+                             int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                             mhd_stream_t stream);
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream);
 int mhd_transpose13_async(float *u, float *b, const float *update_u, const float *update_b, int nx,

@@ -209,3 +209,39 @@ def test_fused_advect_fluidx_equals_the_two_calls(gpu, case, precision):
     assert bits_equal(_h(uf), _h(u1))
     if precision == "exact":
         assert bits_equal(_h(bf), d["advectbyzx_b"])
+
+
+# launch shapes of the fused kernels, including one pencil per 256-, 512- and
+# 1024-thread workgroup (nx = 512, 1024, 2048)
+FUSED_SHAPES = [(64, 8, 8), (128, 8, 8), (200, 4, 4), (512, 4, 3), (1024, 3, 4), (2048, 2, 1),
+                (33, 17, 17)]
+
+
+@pytest.mark.parametrize("shape", FUSED_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_advect2_fluidx_equals_the_three_calls(gpu, shape, precision):
+    """advect2_fluidx is bitwise advectbyzx, advectbyzx with the same u, then
+    fluidx (C/mhd.c:93, :95-97); in exact mode also the oracle's three calls."""
+    import torch
+    from heterogeneous_mhd_amd import ops
+    from oracle import Oracle
+    nx, ny, nz = shape
+    print(f"advect2_fluidx {nx}x{ny}x{nz} {precision}", flush=True)
+    u, b = _random_state(nx, ny, nz, seed=nx + 2 * ny)
+    dt = np.float32(0.05)
+    ud, bd = _t(u, gpu), _t(b, gpu)
+    b1 = bd.clone()
+    ops.advectbyzx(ud, b1, nx, ny, nz, float(dt), ud, bd, precision=precision)
+    b2 = b1.clone()
+    ops.advectbyzx(ud, b2, nx, ny, nz, float(dt), ud, b1, precision=precision)
+    u1 = torch.empty_like(ud)
+    ops.fluidx(u1, b2, nx, ny, nz, float(dt), ud, b2, precision=precision)
+    uf, bf = torch.empty_like(ud), torch.empty_like(bd)
+    ops.advect2_fluidx(uf, bf, nx, ny, nz, float(dt), ud, bd, precision=precision)
+    assert bits_equal(_h(bf), _h(b2))
+    assert bits_equal(_h(uf), _h(u1))
+    if precision == "exact":
+        O = Oracle("f32")
+        ob = O.advectbyzx(u, O.advectbyzx(u, b, nx, ny, nz, dt), nx, ny, nz, dt)
+        assert bits_equal(_h(bf), ob)
+        assert bits_equal(_h(uf), O.fluidx(u, ob, nx, ny, nz, dt))
