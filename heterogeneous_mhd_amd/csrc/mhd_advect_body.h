@@ -88,18 +88,29 @@ __device__ __forceinline__ void advect_group(lds_f *lds, float *b_out, const flo
     float rho[M][5], m1[M][5];
     float bl[M][kLines];  // the advected face field, per line
     float b1m[M];         // main b1 of this pencil
+    // pencil bases are uniform over the wavefront and the in-pencil byte
+    // offset is shared by all of a cell's loads: uniform base + 32-bit
+    // offset, the global_load form with a scalar base
+    const float *ub[5], *bb5[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        ub[q] = uu + op[q] * 5;
+        bb5[q] = bu + op[q] * 3;
+    }
+    const float *bm = b_main + op[1] * 3;
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
-        const int ic = i < nx ? i : nx - 1;
+        const unsigned ic = (unsigned)(i < nx ? i : nx - 1);
+        const unsigned ou = ic * 20u, ob = ic * 12u;  // bytes into a pencil of u, b
 #pragma unroll
         for (int q = 0; q < 5; q++) {
-            rho[r][q] = uu[(op[q] + ic) * 5 + 0];
-            m1[r][q] = uu[(op[q] + ic) * 5 + 1];
+            rho[r][q] = ld_at<float>(ub[q], ou);
+            m1[r][q] = ld_at<float>(ub[q], ou + 4u);
         }
 #pragma unroll
-        for (int t = 0; t < kLines; t++) bl[r][t] = bu[(op[hi[t]] + ic) * 3 + comp[t]];
-        b1m[r] = b_main[(op[1] + ic) * 3 + 0];
+        for (int t = 0; t < kLines; t++) bl[r][t] = ld_at<float>(bb5[hi[t]], ob + 4u * comp[t]);
+        b1m[r] = ld_at<float>(bm, ob);
     }
 #pragma unroll
     for (int r = 0; r < M; r++) {
@@ -268,10 +279,11 @@ __device__ __forceinline__ void advect_group(lds_f *lds, float *b_out, const flo
                 const float b3n = bl[r][2] - (fx[r][2] - fm[2]);
                 const float b1n = (((b1m[r] - fm[0]) + fm[1]) - fm[2]) + fm[3];
                 if (pass == NPASS - 1) {
-                    float *o = b_out + (op[1] + i) * 3;
-                    o[0] = b1n;
-                    o[1] = b2n;
-                    o[2] = b3n;
+                    float *o = b_out + op[1] * 3;
+                    const unsigned ob = 12u * (unsigned)i;
+                    st_at<float>(o, ob, b1n);
+                    st_at<float>(o, ob + 4u, b2n);
+                    st_at<float>(o, ob + 8u, b3n);
                 }
                 keep[r][0] = b1n;
                 keep[r][1] = b2n;

@@ -112,6 +112,19 @@ __device__ __forceinline__ float limiter(float a, float b)
     return r;
 }
 
+// Global access at a byte offset from a wavefront-uniform base: the
+// global_load / global_store form with a scalar base and a 32-bit offset.
+template <class T>
+__device__ __forceinline__ T ld_at(const float *base, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(float *base, unsigned off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 // c*R + 0*R + 0 of the reference, see the file header.
 __device__ __forceinline__ float scale0(float c, float x) { return c * x + (x - x); }
 
@@ -174,19 +187,17 @@ __device__ __forceinline__ void stage_in(lds_f *a_dst, const float *a_src, int n
                                          const float *b_src, int nb, bool vec)
 {
     if (vec) {
-        const v4f *sa = reinterpret_cast<const v4f *>(a_src);
-        const v4f *sb = reinterpret_cast<const v4f *>(b_src);
         const int na4 = na >> 2, nb4 = nb >> 2;
         v4f ra[KA], rb[KB > 0 ? KB : 1];
 #pragma unroll
         for (int k = 0; k < KA; k++) {
             const int f = threadIdx.x + k * BLOCK;
-            ra[k] = sa[f < na4 ? f : na4 - 1];
+            ra[k] = ld_at<v4f>(a_src, 16u * (unsigned)(f < na4 ? f : na4 - 1));
         }
 #pragma unroll
         for (int k = 0; k < KB; k++) {
             const int f = threadIdx.x + k * BLOCK;
-            rb[k] = sb[f < nb4 ? f : nb4 - 1];
+            rb[k] = ld_at<v4f>(b_src, 16u * (unsigned)(f < nb4 ? f : nb4 - 1));
         }
 #pragma unroll
         for (int k = 0; k < KA; k++) {
@@ -261,12 +272,13 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
             }
         }
     } else {
+        const float *bjp = bu + ((size_t)k * ny + jp) * nx * 3, *bkp = bu + ((size_t)kp * ny + j) * nx * 3;
 #pragma unroll
         for (int r = 0; r < M; r++) {
             const int i = lt + r * tp;
             if (live && i < nx) {
-                b2jp[r] = bu[(((size_t)k * ny + jp) * nx + i) * 3 + 1];
-                b3kp[r] = bu[(((size_t)kp * ny + j) * nx + i) * 3 + 2];
+                b2jp[r] = ld_at<float>(bjp, 12u * (unsigned)i + 4u);
+                b3kp[r] = ld_at<float>(bkp, 12u * (unsigned)i + 8u);
             }
         }
         stage_in<BLOCK, KU, KB>(A, gu, nu, B, gb, nb, vec);
@@ -336,12 +348,18 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
             const int ip = i + 1 == nx ? 0 : i + 1;
 #pragma unroll
             for (int c = 0; c < 5; c++)  // flux
-                Bp[5 * i + c] = FAST ? 0.5f * (s[r][c] - Ap[5 * ip + c])
+                Bp[5 * i + c] = FAST ? s[r][c] - Ap[5 * ip + c]  // twice the flux, see hdt
                                      : h * s[r][c] + (-h) * Ap[5 * ip + c] + z;
         }
     }
     __syncthreads();
     const float hdt = (float)(-1.0 * (double)dt / 2.0);
+    // FAST keeps the stage fluxes, the limiter inputs and the final flux at
+    // twice their value and halves the time step instead: power-of-two
+    // scaling is exact and the limiter is homogeneous of degree one, so the
+    // results are the same (outside the denormal and overflow ranges) with
+    // 20 fewer multiplies per cell.
+    const float hdt2 = 0.5f * hdt;
     cmax = 0.0f;
 #pragma unroll
     for (int r = 0; r < M; r++) {
@@ -351,7 +369,7 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
 #pragma unroll
             for (int c = 0; c < 5; c++) {
                 if constexpr (FAST) {
-                    s[r][c] = fmaf(hdt, Bp[5 * i + c] - Bp[5 * im + c], q[r][c]);
+                    s[r][c] = fmaf(hdt2, Bp[5 * i + c] - Bp[5 * im + c], q[r][c]);
                 } else {
                     const float d = Bp[5 * i + c] - Bp[5 * im + c] + z;
                     s[r][c] = q[r][c] + hdt * d + z;           // u1
@@ -405,8 +423,8 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
                 const float fr = s[r][c], frm = Ap[5 * im + c];
                 const float flm = Bp[5 * i + c], fl0 = Bp[5 * ip + c];
                 if constexpr (FAST) {
-                    D[r][c] = 0.5f * (fr - frm);
-                    E[r][c] = 0.5f * (flm - fl0);
+                    D[r][c] = fr - frm;   // twice dfrm
+                    E[r][c] = flm - fl0;  // twice dflm
                     s[r][c] = fr - fl0;
                 } else {
                     D[r][c] = h * fr + (-h) * frm + z;
@@ -439,7 +457,7 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
                 if constexpr (FAST) {
                     const float dfr = limiter_fast(Ap[5 * ip + c], D[r][c]);
                     const float dfl = limiter_fast(Bp[5 * ip + c], E[r][c]);
-                    s[r][c] = 0.5f * (s[r][c] + (dfr - dfl));
+                    s[r][c] = fmaf(0.5f, dfr - dfl, s[r][c]);  // twice the flux
                 } else {
                     const float dfr = limiter(Ap[5 * ip + c], D[r][c]);   // (dfrp, dfrm)
                     const float dfl = limiter(Bp[5 * ip + c], E[r][c]);   // (dflp, dflm)
@@ -460,6 +478,7 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
     }
     __syncthreads();
     const float fdt = (float)(-1.0 * (double)dt);
+    const float fdt2 = 0.5f * fdt;
     float tcf[M];
     int toff[M];
     int lnan = -1;
@@ -474,7 +493,7 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
 #pragma unroll
             for (int c = 0; c < 5; c++) {
                 if constexpr (FAST) {
-                    un[c] = fmaf(fdt, Ap[5 * i + c] - Ap[5 * im + c], q[r][c]);
+                    un[c] = fmaf(fdt2, Ap[5 * i + c] - Ap[5 * im + c], q[r][c]);
                 } else {
                     const float d = Ap[5 * i + c] - Ap[5 * im + c] + z;
                     un[c] = q[r][c] + fdt * d + z;
@@ -492,12 +511,11 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
         float *go = u_out + (size_t)pen0 * nx * 5;
         if (vec) {
             const lds_v4 *s4 = (const lds_v4 *)B;
-            v4f *d4 = reinterpret_cast<v4f *>(go);
             const int n4 = nu >> 2;
 #pragma unroll
             for (int kk = 0; kk < KU; kk++) {
                 const int f = threadIdx.x + kk * BLOCK;
-                if (f < n4) d4[f] = s4[f];
+                if (f < n4) st_at<v4f>(go, 16u * (unsigned)f, s4[f]);
             }
         } else {
             for (int f = threadIdx.x; f < nu; f += BLOCK) go[f] = B[f];
