@@ -26,6 +26,8 @@ import sys
 
 # kernel-name prefix -> operator whose C call launches it
 _OPS = [
+    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, false, \d+, \w+, 2>"), "advect2_fluidx"),
+    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, true, \d+, \w+, 2>"), "advect2_fluidx fast"),
     (re.compile(r"advect_fluid_kernel<\d+, \d+, false, false[,>]"), "advect_fluidx"),
     (re.compile(r"advect_fluid_kernel<\d+, \d+, false, true[,>]"), "advect_fluidx fast"),
     (re.compile(r"advect_fluid_kernel<\d+, \d+, true, false[,>]"), "advect_fluidx+calcfl"),
