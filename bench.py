@@ -175,7 +175,7 @@ def timed_steps(box, steps: int, warmup: int, graph: bool):
 
 def corrector_bench(seeds: int, steps: int):
     """The learned-corrector workload: held-out seeds, 4x-downsampled 256^3
-    runs as the judge; errors of plain 64^3, corrected 64^3 and plain 128^3
+    runs as the reference solution; errors of plain 64^3, corrected 64^3 and plain 128^3
     at 50 and `steps` coarse steps, with their GPU times."""
     from heterogeneous_mhd_amd import corrector
     if not corrector.WEIGHTS.exists():

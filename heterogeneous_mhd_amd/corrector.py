@@ -454,7 +454,7 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
             T = cb.box.time()[0]
             gpu["plain"] += _timed(lambda: pc.advance_to(T))
             gpu["plain_2x"] += _timed(lambda: p2.advance_to(2 * T))
-            fbox.advance_to(4 * T)  # the judge
+            fbox.advance_to(4 * T)  # the reference solution
             with torch.cuda.stream(fbox.stream):
                 uf, bf = downsample(fbox.u, fbox.b, fine, r)
             with torch.cuda.stream(p2.stream):
