@@ -113,9 +113,11 @@ def _random_state(nx, ny, nz, seed, amp_v=0.3, amp_a=0.1):
 
 
 # shapes covering every launch configuration of the sweeps: 1/2/4 cells per
-# thread, 1-4 pencils per workgroup, 512-thread groups for nx > 1024
+# thread, 1-4 pencils per workgroup, 512-thread groups for nx > 1024, and
+# one-pencil workgroups with several rows of pencils
 SHAPES = [(64, 8, 8), (96, 6, 6), (128, 8, 8), (200, 4, 4), (256, 4, 4), (384, 3, 3), (512, 2, 2),
-          (1000, 2, 2), (1024, 2, 2), (1536, 2, 2), (2048, 1, 1), (33, 17, 17)]
+          (1000, 2, 2), (1024, 2, 2), (1536, 2, 2), (2048, 1, 1), (33, 17, 17), (512, 4, 4),
+          (1024, 8, 4)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
