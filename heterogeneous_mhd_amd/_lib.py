@@ -61,6 +61,7 @@ SIGNATURES = {
                                           C.c_float, C.c_int, _vp]),
     "mhd_transpose12_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "mhd_transpose13_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "mhd_transpose13_any_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "mhd_copy_matrix_async": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "mhd_calcfl_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "mhd_calcfl_async": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_double, _vp,

@@ -315,6 +315,12 @@ int mhd_transpose13_async(float *u, float *b, const float *update_u, const float
     return mhd::launch_transpose13(u, b, update_u, update_b, nx, ny, nz, (hipStream_t)stream);
 }
 
+int mhd_transpose13_any_async(float *u, float *b, const float *update_u, const float *update_b,
+                              int nx, int ny, int nz, mhd_stream_t stream)
+{
+    return mhd::launch_transpose13(u, b, update_u, update_b, nx, ny, nz, (hipStream_t)stream, true);
+}
+
 int mhd_copy_matrix_async(const float *u, const float *b, int nx, int ny, int nz, float *update_u,
                           float *update_b, mhd_stream_t stream)
 {
@@ -364,8 +370,9 @@ int mhd_plan_create(mhd_plan **plan, int nx, int ny, int nz, double cfl, float t
                     mhd_stream_t stream)
 {
     if (!plan || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
-    if (ny != nz) return MHD_ERR_SHAPE;
-    if (nx > mhd::kMaxPencil || ny > mhd::kMaxPencil) return MHD_ERR_TOO_LONG;
+    // every axis is a sweep's pencil once; ny != nz boxes use the general
+    // transpose13 (the reference's needs ny == nz at C/mhd.c:88, :102)
+    if (nx > mhd::kMaxPencil || ny > mhd::kMaxPencil || nz > mhd::kMaxPencil) return MHD_ERR_TOO_LONG;
     mhd_plan *p = new mhd_plan();
     p->nx = nx, p->ny = ny, p->nz = nz, p->cfl = cfl, p->tf = tf, p->s = (hipStream_t)stream;
     const size_t wsb = mhd::cfl_workspace_bytes(nx, ny, nz);
@@ -478,13 +485,13 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
     // y sweep, dims (ny, nx, nz)
     RUN(mhd::launch_fluidx(U0, U0, B0, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :80
     RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, ny, nx, nz, dt, 0.0f, s, f));               // :84
-    RUN(mhd::launch_transpose13(U1, B0, U0, B1, ny, nx, nz, s));                         // :88
+    RUN(mhd::launch_transpose13(U1, B0, U0, B1, ny, nx, nz, s, true));                   // :88
     // z sweeps, dims (nz, nx, ny)
     RUN(mhd::launch_fluidx(U1, U1, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :91
     // :93 advectbyzx and the second half's first pair (:95-97) in one kernel
     RUN(mhd::launch_advect_fluid(U0, B1, U1, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr,
                                  0.0, f, 2));                                            // :93-97
-    RUN(mhd::launch_transpose13(U1, B2, U0, B1, nz, nx, ny, s));                         // :102
+    RUN(mhd::launch_transpose13(U1, B2, U0, B1, nz, nx, ny, s, true));                   // :102
     // y again, dims (ny, nx, nz)
     RUN(mhd::launch_advect_fluid(U0, B1, U1, B2, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr,
                                  0.0, f));                                               // :105-107

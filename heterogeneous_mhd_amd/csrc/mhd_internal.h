@@ -114,8 +114,10 @@ int launch_advect_fluid(float *u_out, float *b_out, const float *uu, const float
 int launch_transpose12(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                        int nz, hipStream_t s);
 
+// any_shape: nx != nz allowed (out dims (nz, ny, nx)); the reference's
+// transpose13 refuses it, so its C-ABI entry points keep any_shape false.
 int launch_transpose13(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
-                       int nz, hipStream_t s);
+                       int nz, hipStream_t s, bool any_shape = false);
 
 // CFL reduction workspace: a ticket counter + one record per workgroup.  The
 // counter must be zero before the first launch; the kernels leave it zero.

@@ -66,10 +66,10 @@ int launch_transpose12(float *u_out, float *b_out, const float *uu, const float 
 }
 
 int launch_transpose13(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
-                       int nz, hipStream_t s)
+                       int nz, hipStream_t s, bool any_shape)
 {
     if (!u_out || !b_out || !uu || !bu || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
-    if (nx != nz) return MHD_ERR_SHAPE;
+    if (nx != nz && !any_shape) return MHD_ERR_SHAPE;  // C/transpose13.c:16
     // i = x, a = z, plane index = y.  in cell = i + z*nx*ny + y*nx;
     // out dims (nz, ny, nx): out cell = z + y*nz + i*nz*ny.
     const size_t sa_in = (size_t)nx * ny, sz_in = (size_t)nx;

@@ -111,6 +111,17 @@ def transpose13(u, b, update_u, update_b, nx, ny, nz):
           "transpose13")
 
 
+def transpose13_any(u, b, update_u, update_b, nx, ny, nz):
+    """x<->z swap of update_* into u, b for any nx, nz: out dims (nz, ny, nx),
+    the component permutation of C/transpose13.c:24-31 (which itself refuses
+    nx != nz)."""
+    n = nx * ny * nz
+    check(lib().mhd_transpose13_any_async(_dev(u, 5 * n, "u"), _dev(b, 3 * n, "b"),
+                                          _dev(update_u, 5 * n, "update_u"),
+                                          _dev(update_b, 3 * n, "update_b"), nx, ny, nz,
+                                          _stream()), "transpose13_any")
+
+
 def copy_matrix(u, b, nx, ny, nz, update_u, update_b):
     """update <- main (C/copy_matrix.c:9-38)."""
     n = nx * ny * nz

@@ -83,8 +83,6 @@ class Box:
 
     def __init__(self, nx: int, ny: int, nz: int, cfl: float = 0.7, tf: float = 0.0,
                  device="cuda", stream: torch.cuda.Stream | None = None, precision: str = "exact"):
-        if ny != nz:
-            raise MHDError("the split step needs ny == nz (transpose13 swaps y and z)")
         self.nx, self.ny, self.nz = nx, ny, nz
         self.n = nx * ny * nz
         self.device = torch.device(device)

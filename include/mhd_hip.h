@@ -67,7 +67,7 @@ float calcfl(float *u, float *b, int *nx, int *ny, int *nz);
 
 #define MHD_OK 0
 #define MHD_ERR_ARG -1       /* bad dimensions or null pointer */
-#define MHD_ERR_SHAPE -2     /* transpose13 with nx != nz; step with ny != nz */
+#define MHD_ERR_SHAPE -2     /* transpose13 (the reference form) with nx != nz */
 #define MHD_ERR_HIP -3       /* a HIP runtime call failed */
 #define MHD_ERR_TOO_LONG -4  /* pencil longer than mhd_max_pencil() */
 #define MHD_ERR_NOMEM -5     /* device allocation failed */
@@ -127,6 +127,12 @@ int mhd_transpose12_async(float *u, float *b, const float *update_u, const float
                           int ny, int nz, mhd_stream_t stream);
 int mhd_transpose13_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream);
+/* transpose13 for any nx, nz: out dims (nz, ny, nx) with the same component
+ * permutation.  No reference counterpart: C/transpose13.c:16 refuses nx != nz
+ * (so the box of C/mhd.c needs ny == nz); the device plan uses this one, so it
+ * also steps boxes with ny != nz. */
+int mhd_transpose13_any_async(float *u, float *b, const float *update_u, const float *update_b,
+                              int nx, int ny, int nz, mhd_stream_t stream);
 int mhd_copy_matrix_async(const float *u, const float *b, int nx, int ny, int nz, float *update_u,
                           float *update_b, mhd_stream_t stream);
 
@@ -144,10 +150,12 @@ int mhd_calcfl_async(float *inv_c_dev, const float *u, const float *b, int nx, i
 /* ---- full step on the device (the sweep order of C/mhd.c:59-124) ---- */
 typedef struct mhd_plan mhd_plan;
 
-/* A plan for boxes of nx*ny*nz (ny == nz, as transpose13 requires at the
- * driver's call sites).  cfl is the CFL factor (the reference uses 0.9);
- * tf > 0 enables the reference's end-time clamp.  Owns the device scalars
- * t and dt and the CFL workspace. */
+/* A plan for periodic boxes of nx*ny*nz, every axis <= 2048.  The reference
+ * needs ny == nz (its transpose13 at the driver's call sites); the plan uses
+ * mhd_transpose13_any and also takes ny != nz.  cfl is the CFL factor (the
+ * reference uses 0.9); tf > 0 enables the reference's end-time clamp.  Owns
+ * the device scalars t and dt, the CFL workspace and a third face-field
+ * buffer (12 bytes per cell). */
 int mhd_plan_create(mhd_plan **plan, int nx, int ny, int nz, double cfl, float tf,
                     mhd_stream_t stream);
 void mhd_plan_destroy(mhd_plan *plan);

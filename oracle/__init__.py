@@ -47,6 +47,8 @@ class Oracle:
         L.mhdref_transpose12.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
         L.mhdref_transpose13.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
         L.mhdref_transpose13.restype = C.c_int
+        L.mhdref_transpose13_any.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
+        L.mhdref_transpose13_any.restype = None
         L.mhdref_calcfl.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_int]
         L.mhdref_calcfl.restype = R
         L.mhdref_cfl_cell.argtypes = [_vp, _vp]
@@ -54,6 +56,9 @@ class Oracle:
         L.mhdref_step.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, R,
                                   C.POINTER(R)]
         L.mhdref_step.restype = R
+        L.mhdref_step_any.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, R,
+                                  C.POINTER(R)]
+        L.mhdref_step_any.restype = R
         L.mhdref_diagnostics.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_int,
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.mhdref_set_threads(threads)
@@ -91,6 +96,13 @@ class Oracle:
             raise ValueError("transpose13 needs nx == nz")
         return u, b
 
+    def transpose13_any(self, ub, bb, nx, ny, nz):
+        """x<->z swap for any nx, nz (the reference refuses nx != nz)."""
+        ub, bb = self._a(ub), self._a(bb)
+        u, b = np.empty_like(ub), np.empty_like(bb)
+        self.lib.mhdref_transpose13_any(_ptr(u), _ptr(b), _ptr(ub), _ptr(bb), nx, ny, nz)
+        return u, b
+
     def calcfl(self, u, b, nx, ny, nz):
         u, b = self._a(u), self._a(b)
         return self.dtype(self.lib.mhdref_calcfl(_ptr(u), _ptr(b), nx, ny, nz))
@@ -99,15 +111,17 @@ class Oracle:
         q, bb = self._a(q), self._a(bb)
         return self.dtype(self.lib.mhdref_cfl_cell(_ptr(q), _ptr(bb)))
 
-    def run(self, u, b, nx, ny, nz, nsteps, cfl=0.7, tf=0.0, t=0.0, on_step=None):
-        """nsteps full steps; returns (u, b, t, [dt per step])."""
+    def run(self, u, b, nx, ny, nz, nsteps, cfl=0.7, tf=0.0, t=0.0, on_step=None, any_shape=False):
+        """nsteps full steps; returns (u, b, t, [dt per step]).  any_shape:
+        boxes with ny != nz, through transpose13_any."""
         u, b = self._a(u).copy(), self._a(b).copy()
         uu, bu = u.copy(), b.copy()
         tt = self.R(t)
         dts = []
+        step = self.lib.mhdref_step_any if any_shape else self.lib.mhdref_step
         for s in range(nsteps):
-            dt = self.lib.mhdref_step(_ptr(u), _ptr(b), _ptr(uu), _ptr(bu), nx, ny, nz, float(cfl),
-                                      self.R(tf), C.byref(tt))
+            dt = step(_ptr(u), _ptr(b), _ptr(uu), _ptr(bu), nx, ny, nz, float(cfl), self.R(tf),
+                      C.byref(tt))
             dts.append(self.dtype(dt))
             if on_step is not None:
                 on_step(s, u, b, dts[-1])

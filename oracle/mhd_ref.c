@@ -26,6 +26,10 @@
  *   calcfl            golden vectors + closed form on a uniform state +
  *                     brute force on tiny boxes
  *   step driver       per-step SHA-256 and dt of golden trajectories
+ *   transpose13_any,  (no reference counterpart: boxes with ny != nz)
+ *   step_any          closed-form index formula + involution; equal to
+ *                     transpose13 / the reference step where ny == nz;
+ *                     conserved mass and energy, div.B at round-off
  */
 #include <math.h>
 #include <stdint.h>
@@ -335,13 +339,24 @@ void mhdref_transpose12(real *u, real *b, const real *uu, const real *bu, int nx
             }
 }
 
+void mhdref_transpose13_any(real *u, real *b, const real *uu, const real *bu, int nx, int ny, int nz);
+
 /* transpose13 (C/transpose13.c:9-62): needs nx==nz; out dims (nz,ny,nx); */
 /* u comps (1,4,3,2,5), b comps (3,2,1).  Returns -1 (and writes nothing) */
 /* when nx != nz, where the reference prints and returns.                 */
 int mhdref_transpose13(real *u, real *b, const real *uu, const real *bu, int nx, int ny, int nz)
 {
-    static const int pu[5] = {0, 3, 2, 1, 4}, pb[3] = {2, 1, 0};
     if (nx != nz) return -1;
+    mhdref_transpose13_any(u, b, uu, bu, nx, ny, nz);
+    return 0;
+}
+
+/* The same x<->z swap for any nx, nz (no reference counterpart: the      */
+/* reference refuses nx != nz).  Closed-form permutation; pinned by the   */
+/* index formula and by transpose13_any twice being the identity.         */
+void mhdref_transpose13_any(real *u, real *b, const real *uu, const real *bu, int nx, int ny, int nz)
+{
+    static const int pu[5] = {0, 3, 2, 1, 4}, pb[3] = {2, 1, 0};
 #pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int j = 0; j < ny; j++)
         for (int k = 0; k < nz; k++)
@@ -349,7 +364,6 @@ int mhdref_transpose13(real *u, real *b, const real *uu, const real *bu, int nx,
                 for (int c = 0; c < 5; c++) U5(u, nz, ny, c, k, j, i) = U5(uu, nx, ny, pu[c], i, j, k);
                 for (int c = 0; c < 3; c++) B3(b, nz, ny, c, k, j, i) = B3(bu, nx, ny, pb[c], i, j, k);
             }
-    return 0;
 }
 
 /* max_3num (C/calcfl.c:65-89) */
@@ -405,8 +419,25 @@ void mhdref_copy_matrix(const real *u, const real *b, int nx, int ny, int nz, re
 /* reference's clamp dt <= (tf-t)/2 applies; *t advances by 2dt.          */
 /* Returns dt.                                                            */
 /* --------------------------------------------------------------------- */
+static real step_body(real *u, real *b, real *uu, real *bu, int nx, int ny, int nz, double cfl,
+                      real tf, real *t, int any_shape);
+
 real mhdref_step(real *u, real *b, real *uu, real *bu, int nx, int ny, int nz, double cfl,
                  real tf, real *t)
+{
+    return step_body(u, b, uu, bu, nx, ny, nz, cfl, tf, t, 0);
+}
+
+/* The same step for ny != nz, with transpose13_any (no reference          */
+/* counterpart: the reference's transpose13 refuses such boxes).           */
+real mhdref_step_any(real *u, real *b, real *uu, real *bu, int nx, int ny, int nz, double cfl,
+                     real tf, real *t)
+{
+    return step_body(u, b, uu, bu, nx, ny, nz, cfl, tf, t, 1);
+}
+
+static real step_body(real *u, real *b, real *uu, real *bu, int nx, int ny, int nz, double cfl,
+                      real tf, real *t, int any_shape)
 {
     real dt = (real)(cfl * (double)mhdref_calcfl(u, b, nx, ny, nz));  /* :62 */
     if (tf > 0 && dt > (tf - *t) / 2) dt = (tf - *t) / 2;              /* :63 */
@@ -417,12 +448,16 @@ real mhdref_step(real *u, real *b, real *uu, real *bu, int nx, int ny, int nz, d
     mhdref_transpose12(u, b, uu, bu, nx, ny, nz);           CP(ny, nx, nz);   /* :78-79 */
     mhdref_fluidx(u, b, ny, nx, nz, dt, uu, bu);            CP(ny, nx, nz);   /* :80-81 */
     mhdref_advectbyzx(b, ny, nx, nz, dt, uu, bu);           CP(ny, nx, nz);   /* :84-85 */
-    mhdref_transpose13(u, b, uu, bu, ny, nx, nz);           CP(nz, nx, ny);   /* :88-89 */
+    if (any_shape) mhdref_transpose13_any(u, b, uu, bu, ny, nx, nz);
+    else mhdref_transpose13(u, b, uu, bu, ny, nx, nz);
+    CP(nz, nx, ny);                                                            /* :88-89 */
     mhdref_fluidx(u, b, nz, nx, ny, dt, uu, bu);            CP(nz, nx, ny);   /* :91-92 */
     mhdref_advectbyzx(b, nz, nx, ny, dt, uu, bu);           CP(nz, nx, ny);   /* :93-94 */
     mhdref_advectbyzx(b, nz, nx, ny, dt, uu, bu);           CP(nz, nx, ny);   /* :95-96 */
     mhdref_fluidx(u, b, nz, nx, ny, dt, uu, bu);            CP(nz, nx, ny);   /* :97-98 */
-    mhdref_transpose13(u, b, uu, bu, nz, nx, ny);           CP(ny, nx, nz);   /* :102-103 */
+    if (any_shape) mhdref_transpose13_any(u, b, uu, bu, nz, nx, ny);
+    else mhdref_transpose13(u, b, uu, bu, nz, nx, ny);
+    CP(ny, nx, nz);                                                            /* :102-103 */
     mhdref_advectbyzx(b, ny, nx, nz, dt, uu, bu);           CP(ny, nx, nz);   /* :105-106 */
     mhdref_fluidx(u, b, ny, nx, nz, dt, uu, bu);            CP(ny, nx, nz);   /* :107-108 */
     mhdref_transpose12(u, b, uu, bu, ny, nx, nz);           CP(nx, ny, nz);   /* :112-113 */
