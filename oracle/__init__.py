@@ -42,6 +42,7 @@ class Oracle:
         self.R = R
         L = self.lib
         L.mhdref_set_threads.argtypes = [C.c_int]
+        L.mhdref_set_cuda_semantics.argtypes = [C.c_int]
         L.mhdref_fluidx.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_int, R, _vp, _vp]
         L.mhdref_advectbyzx.argtypes = [_vp, C.c_int, C.c_int, C.c_int, R, _vp, _vp]
         L.mhdref_transpose12.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
@@ -95,6 +96,12 @@ class Oracle:
         if rc != 0:
             raise ValueError("transpose13 needs nx == nz")
         return u, b
+
+    def set_cuda_semantics(self, on: bool):
+        """Switch this library between the reference's C path (default) and
+        the semantics of its CUDA path (the B1 neighbour index and the
+        face-averaged CFL); process-wide for this precision's library."""
+        self.lib.mhdref_set_cuda_semantics(1 if on else 0)
 
     def transpose13_any(self, ub, bb, nx, ny, nz):
         """x<->z swap for any nx, nz (the reference refuses nx != nz)."""

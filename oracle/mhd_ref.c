@@ -26,6 +26,9 @@
  *   calcfl            golden vectors + closed form on a uniform state +
  *                     brute force on tiny boxes
  *   step driver       per-step SHA-256 and dt of golden trajectories
+ *   CUDA semantics    (mhdref_set_cuda_semantics) the two index/averaging
+ *                     changes only; checked to differ from the C path in
+ *                     exactly the fields and steps they touch
  *   transpose13_any,  (no reference counterpart: boxes with ny != nz)
  *   step_any          closed-form index formula + involution; equal to
  *                     transpose13 / the reference step where ny == nz;
@@ -51,6 +54,18 @@ typedef REAL real;
 static int g_threads = 1;
 
 void mhdref_set_threads(int t) { g_threads = t < 1 ? 1 : t; }
+
+/* The reference's CUDA path (Cuda/) instead of its C path, for the
+ * cross-check of how far the two drift apart.  Two semantic divergences:
+ *  - advectbyzx2_kernel reads the upwind pencil's rho, m1 at (i, km, k), the
+ *    j slot, instead of (i, j, km) (Cuda/cuda_advectbyzx.cu:204,212 against
+ *    C/advectbyzxB1.c:34), i.e. the B1 line's neighbour is pencil (k-1, k);
+ *  - cfl_kernel uses the face-averaged B in ps and p (Cuda/cuda_cfl.cu:102-121)
+ *    where C uses the face values (C/calcfl.c:50-54).
+ * Its other defects (uninitialised cfl_c entries, host-side leaks) are not
+ * reproducible semantics.  Off by default. */
+static int g_cuda = 0;
+void mhdref_set_cuda_semantics(int on) { g_cuda = on != 0; }
 int mhdref_real_size(void) { return (int)sizeof(real); }
 
 /* --------------------------------------------------------------------- */
@@ -308,6 +323,10 @@ void mhdref_advectbyzx(real *b, int nx, int ny, int nz, real dt, const real *uu,
                 int j = (int)(pk % ny), k = (int)(pk / ny);
                 int jo = pass == 0 ? (j + ny - 1) % ny : j;          /* A1 :30 */
                 int ko = pass == 0 ? k : (k + nz - 1) % nz;          /* B1 :30 */
+                if (g_cuda && pass == 1) {          /* Cuda/cuda_advectbyzx.cu:204 */
+                    jo = ((k + nz - 1) % nz) % ny;
+                    ko = k;
+                }
                 ref_advect_pencil(b, nx, ny, dt, uu, bu, tmp, j, k, jo, ko, comp, scratch);
             }
             free(scratch);
@@ -398,7 +417,17 @@ real mhdref_calcfl(const real *u, const real *b, int nx, int ny, int nz)
     real c = (real)0.0;
     size_t ncell = (size_t)nx * ny * nz;
     for (size_t s = 0; s < ncell; s++) {
-        real t = mhdref_cfl_cell(u + 5 * s, b + 3 * s);
+        real t;
+        if (g_cuda) {  /* face-averaged B, Cuda/cuda_cfl.cu:102-108 */
+            const int i = (int)(s % nx), j = (int)((s / nx) % ny), k = (int)(s / ((size_t)nx * ny));
+            real ba[3];
+            ba[0] = (real)((double)(B3(b, nx, ny, 0, i, j, k) + B3(b, nx, ny, 0, (i + 1) % nx, j, k)) / 2.0);
+            ba[1] = (real)((double)(B3(b, nx, ny, 1, i, j, k) + B3(b, nx, ny, 1, i, (j + 1) % ny, k)) / 2.0);
+            ba[2] = (real)((double)(B3(b, nx, ny, 2, i, j, k) + B3(b, nx, ny, 2, i, j, (k + 1) % nz)) / 2.0);
+            t = mhdref_cfl_cell(u + 5 * s, ba);
+        } else {
+            t = mhdref_cfl_cell(u + 5 * s, b + 3 * s);
+        }
         c = c > t ? c : t;                                         /* max_2num :93-100 */
     }
     return 1 / c;                                                  /* :61 */
