@@ -207,12 +207,17 @@ def main():
     ap.add_argument("--corrector-seeds", type=int, default=2)
     ap.add_argument("--corrector-steps", type=int, default=100)
     ap.add_argument("--precision", choices=["exact", "fast"], default="fast")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="print only the 96-box ensemble measurement (for profiling it)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the z-slab "
               "decomposition with all-to-all transposes is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench(precision=args.precision)), flush=True)
+        return
     import torch
     from heterogeneous_mhd_amd import solver
     n = args.n

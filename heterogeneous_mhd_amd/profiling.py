@@ -44,6 +44,15 @@ _OPS = [
     (re.compile(r"cfl_fold_kernel"), "calcfl fold"),
     (re.compile(r"begin_step_kernel"), "begin_step"),
     (re.compile(r"copyBuffer"), "copy"),
+    (re.compile(r"ens_fluid_kernel<\d+, \d+, \d+, true"), "ensemble fluidx+calcfl"),
+    (re.compile(r"ens_fluid_kernel"), "ensemble fluidx"),
+    (re.compile(r"ens_advect_kernel"), "ensemble advectbyzx"),
+    (re.compile(r"ens_fused_kernel<\d+, \d+, \d+, true"), "ensemble advect_fluidx+calcfl"),
+    (re.compile(r"ens_fused_kernel"), "ensemble advect_fluidx"),
+    (re.compile(r"ens_transpose_kernel<5>"), "ensemble transpose (u)"),
+    (re.compile(r"ens_transpose_kernel<3>"), "ensemble transpose (b)"),
+    (re.compile(r"ens_cfl_fold_kernel"), "ensemble calcfl fold"),
+    (re.compile(r"ens_begin_kernel"), "ensemble begin_step"),
 ]
 
 
