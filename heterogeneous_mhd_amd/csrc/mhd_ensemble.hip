@@ -58,12 +58,12 @@ __device__ __forceinline__ int find_box(const int *pre, int nbox, int item)
 
 template <int BLOCK, int M, int TP, bool CFL, bool FAST>
 __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *Uout, const float *Uin,
-                                                          const float *Bf, void *ws)
+                                                          const float *Bf, void *ws, int first,
+                                                          int end)
 {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float red[2 * (BLOCK / 64)];
-    const int total = E.gpre[E.nbox];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    for (int item = first + blockIdx.x; item < end; item += gridDim.x) {
         const int box = find_box(E.gpre, E.nbox, item);
         const int n = E.n[box];
         __syncthreads();  // the previous item's last LDS reads are done
@@ -76,11 +76,11 @@ __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *Uout,
 
 template <int BLOCK, int M, int TP, bool FAST>
 __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout, const float *Bmain,
-                                                           const float *U, const float *Bin)
+                                                           const float *U, const float *Bin, int first,
+                                                           int end)
 {
     extern __shared__ float lds[];
-    const int total = E.gpre[E.nbox];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    for (int item = first + blockIdx.x; item < end; item += gridDim.x) {
         const int box = find_box(E.gpre, E.nbox, item);
         const int n = E.n[box];
         const long long bo = E.boff[box];
@@ -95,12 +95,12 @@ __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout
 // to Uout, which must not be Uin
 template <int BLOCK, int M, int TP, bool CFL, bool FAST>
 __global__ __launch_bounds__(BLOCK) void ens_fused_kernel(EnsDev E, float *Uout, float *Bout,
-                                                          const float *Uin, const float *Bin, void *ws)
+                                                          const float *Uin, const float *Bin, void *ws,
+                                                          int first, int end)
 {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     __shared__ float red[2 * (BLOCK / 64)];
-    const int total = E.gpre[E.nbox];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    for (int item = first + blockIdx.x; item < end; item += gridDim.x) {
         const int box = find_box(E.gpre, E.nbox, item);
         const int n = E.n[box];
         const long long bo = E.boff[box], uo = E.uoff[box];
@@ -191,40 +191,40 @@ constexpr int kNumGrids = sizeof(kGridMult) / sizeof(kGridMult[0]);
 
 template <int BLOCK, int M, int TP, bool CFL, bool FAST>
 int launch_fluid_k(const EnsDev &E, float *Uo, const float *Ui, const float *B, size_t lds, void *ws,
-                   int grid, hipStream_t s)
+                   int grid, hipStream_t s, int first, int end)
 {
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, TP, CFL, FAST>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
     hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, TP, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
-                       Uo, Ui, B, ws);
+                       Uo, Ui, B, ws, first, end);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
 template <int BLOCK, int M, int TP, bool FAST>
 int launch_advect_k(const EnsDev &E, float *Bo, const float *Bm, const float *U, const float *Bi,
-                    size_t lds, int grid, hipStream_t s)
+                    size_t lds, int grid, hipStream_t s, int first, int end)
 {
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void *)ens_advect_kernel<BLOCK, M, TP, FAST>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
     hipLaunchKernelGGL((ens_advect_kernel<BLOCK, M, TP, FAST>), dim3(grid), dim3(BLOCK), lds, s, E, Bo,
-                       Bm, U, Bi);
+                       Bm, U, Bi, first, end);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
 template <int BLOCK, int M, int TP, bool CFL, bool FAST>
 int launch_fused_k(const EnsDev &E, float *Uo, float *Bo, const float *Ui, const float *Bi, size_t lds,
-                   void *ws, int grid, hipStream_t s)
+                   void *ws, int grid, hipStream_t s, int first, int end)
 {
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void *)ens_fused_kernel<BLOCK, M, TP, CFL, FAST>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
     hipLaunchKernelGGL((ens_fused_kernel<BLOCK, M, TP, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
-                       Uo, Bo, Ui, Bi, ws);
+                       Uo, Bo, Ui, Bi, ws, first, end);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -232,42 +232,69 @@ int launch_fused_k(const EnsDev &E, float *Uo, float *Bo, const float *Ui, const
 // 1: advect (B0 -> Bo with b_main Bm, u U); 2: fused (Uo, Bo <- Ui, Bi).
 template <int BLOCK, int M, int TP>
 int launch_sweep(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo, const float *Ui,
-                 const float *Bi, const float *Bm, int nmax, void *ws, int grid, hipStream_t s)
+                 const float *Bi, const float *Bm, int nmax, void *ws, int grid, hipStream_t s, int first,
+                 int end)
 {
     constexpr int pb = BLOCK / TP;
     if (kind == 1) {
         const size_t lds = sizeof(float) * 2 * kLines * (size_t)nmax * pb;
-        return fast ? launch_advect_k<BLOCK, M, TP, true>(E, Bo, Bm, Ui, Bi, lds, grid, s)
-                    : launch_advect_k<BLOCK, M, TP, false>(E, Bo, Bm, Ui, Bi, lds, grid, s);
+        return fast ? launch_advect_k<BLOCK, M, TP, true>(E, Bo, Bm, Ui, Bi, lds, grid, s, first, end)
+                    : launch_advect_k<BLOCK, M, TP, false>(E, Bo, Bm, Ui, Bi, lds, grid, s, first, end);
     }
     if (kind == 0) {
         const size_t lds = sizeof(float) * 10 * (size_t)nmax * pb;
         if (cfl)
-            return fast ? launch_fluid_k<BLOCK, M, TP, true, true>(E, Uo, Ui, Bi, lds, ws, grid, s)
-                        : launch_fluid_k<BLOCK, M, TP, true, false>(E, Uo, Ui, Bi, lds, ws, grid, s);
-        return fast ? launch_fluid_k<BLOCK, M, TP, false, true>(E, Uo, Ui, Bi, lds, ws, grid, s)
-                    : launch_fluid_k<BLOCK, M, TP, false, false>(E, Uo, Ui, Bi, lds, ws, grid, s);
+            return fast ? launch_fluid_k<BLOCK, M, TP, true, true>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end)
+                        : launch_fluid_k<BLOCK, M, TP, true, false>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end);
+        return fast ? launch_fluid_k<BLOCK, M, TP, false, true>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end)
+                    : launch_fluid_k<BLOCK, M, TP, false, false>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end);
     }
     const size_t lds = sizeof(float) * 10 * (size_t)nmax * pb;  // >= the advect's 2*kLines
     if (cfl)
-        return fast ? launch_fused_k<BLOCK, M, TP, true, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s)
-                    : launch_fused_k<BLOCK, M, TP, true, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s);
-    return fast ? launch_fused_k<BLOCK, M, TP, false, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s)
-                : launch_fused_k<BLOCK, M, TP, false, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s);
+        return fast ? launch_fused_k<BLOCK, M, TP, true, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end)
+                    : launch_fused_k<BLOCK, M, TP, true, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end);
+    return fast ? launch_fused_k<BLOCK, M, TP, false, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end)
+                : launch_fused_k<BLOCK, M, TP, false, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end);
+}
+
+// The items of boxes that fit one cell per thread (n <= TP) form the tail of
+// the work list (boxes are packed by decreasing size); for a two-cell shape
+// they run as a second launch of the one-cell shape with the same block and
+// threads per pencil, so the items, groups and CFL records are the same and
+// no thread of a small box's pencil is idle.  split = the first such item
+// (total: none); nsmall = their largest side.
+template <int BLOCK, int M, int TP>
+int sweep_split(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo, const float *Ui,
+                const float *Bi, const float *Bm, int nmax, int nsmall, int split, int total, void *ws,
+                int grid, hipStream_t s)
+{
+    if (M == 1 || split >= total)
+        return launch_sweep<BLOCK, M, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s, 0,
+                                          total);
+    int rc = MHD_OK;
+    if (split > 0)
+        rc = launch_sweep<BLOCK, M, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s, 0, split);
+    if (rc == MHD_OK)
+        rc = launch_sweep<BLOCK, 1, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nsmall, ws, grid, s, split,
+                                        total);
+    return rc;
 }
 
 int sweep(int shape, int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo,
-          const float *Ui, const float *Bi, const float *Bm, int nmax, void *ws, int grid, hipStream_t s)
+          const float *Ui, const float *Bi, const float *Bm, int nmax, int nsmall, int split, int total,
+          void *ws, int grid, hipStream_t s)
 {
+#define SW(B, M_, T) sweep_split<B, M_, T>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, nsmall, split, total, ws, grid, s)
     switch (shape) {  // the entries of kShapes
-    case 0: return launch_sweep<256, 2, 64>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
-    case 1: return launch_sweep<128, 2, 64>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
-    case 2: return launch_sweep<256, 1, 128>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
-    case 3: return launch_sweep<512, 2, 64>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
-    case 4: return launch_sweep<256, 2, 128>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
-    case 5: return launch_sweep<512, 2, 256>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
-    default: return launch_sweep<1024, 2, 512>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s);
+    case 0: return SW(256, 2, 64);
+    case 1: return SW(128, 2, 64);
+    case 2: return SW(256, 1, 128);
+    case 3: return SW(512, 2, 64);
+    case 4: return SW(256, 2, 128);
+    case 5: return SW(512, 2, 256);
+    default: return SW(1024, 2, 512);
     }
+#undef SW
 }
 
 const int kPU12[5] = {0, 2, 1, 3, 4}, kPB12[3] = {1, 0, 2};
@@ -308,6 +335,7 @@ struct mhd_ensemble {
     std::vector<long long> uoff, boff;  // per box (caller's order)
     size_t ufloats, bfloats;
     int shape, grid;                    // current launch shape and persistent grid size
+    int split, nsmall, total;           // see sweep_split (set with the shape)
     bool fast;                          // FP32 tolerance mode of the sweeps
     // device tables
     void *tab;
@@ -338,6 +366,7 @@ int ens_step(mhd_ensemble *e)
     const hipStream_t s = e->s;
     float *U0 = e->U0, *B0 = e->B0, *U1 = e->U1, *B1 = e->B1;
     const int sh = e->shape, g = e->grid, nm = e->nmax;
+    const int ns = e->nsmall, sp = e->split, tot = e->total;
     const bool f = e->fast;
     int rc = MHD_OK;
 #define RUN(x)                      \
@@ -349,19 +378,19 @@ int ens_step(mhd_ensemble *e)
     RUN(ens_error(hipGetLastError()));
     // the order of C/mhd.c:71-117; the second half's advectbyzx + fluidx pairs
     // are fused, and u ping-pongs so that the step ends in (U0, B0)
-    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U0, B0, nullptr, nm, nullptr, g, s));   // :71
-    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, nullptr, g, s));        // :74
+    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U0, B0, nullptr, nm, ns, sp, tot, nullptr, g, s));   // :71
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, ns, sp, tot, nullptr, g, s));        // :74
     RUN(launch_transpose(E, U0, B0, U1, B1, false, g, s));                             // :78
-    RUN(sweep(sh, 0, false, f, E, U0, nullptr, U0, B0, nullptr, nm, nullptr, g, s));   // :80
-    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U0, B0, B0, nm, nullptr, g, s));        // :84
+    RUN(sweep(sh, 0, false, f, E, U0, nullptr, U0, B0, nullptr, nm, ns, sp, tot, nullptr, g, s));   // :80
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U0, B0, B0, nm, ns, sp, tot, nullptr, g, s));        // :84
     RUN(launch_transpose(E, U1, B0, U0, B1, true, g, s));                              // :88
-    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U1, B0, nullptr, nm, nullptr, g, s));   // :91
-    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, nullptr, g, s));        // :93
-    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, nullptr, g, s));        // :95-97
+    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U1, B0, nullptr, nm, ns, sp, tot, nullptr, g, s));   // :91
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, ns, sp, tot, nullptr, g, s));        // :93
+    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, ns, sp, tot, nullptr, g, s));        // :95-97
     RUN(launch_transpose(E, U1, B1, U0, B0, true, g, s));                              // :102
-    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, nullptr, g, s));        // :105-107
+    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, ns, sp, tot, nullptr, g, s));        // :105-107
     RUN(launch_transpose(E, U1, B1, U0, B0, false, g, s));                             // :112
-    RUN(sweep(sh, 2, true, f, E, U0, B0, U1, B1, nullptr, nm, e->ws, g, s));           // :115-117 + next calcfl
+    RUN(sweep(sh, 2, true, f, E, U0, B0, U1, B1, nullptr, nm, ns, sp, tot, e->ws, g, s));           // :115-117 + next calcfl
     if (rc == MHD_OK) {
         hipLaunchKernelGGL(ens_cfl_fold_kernel, dim3(e->nbox), dim3(kFoldBlock), 0, s, E, e->ws, e->cfl);
         rc = ens_error(hipGetLastError());
@@ -533,6 +562,18 @@ int mhd_ensemble_set_config(mhd_ensemble *e, int shape, int grid)
     }
     hipError_t er = hipMemcpy((void *)e->E.gpre, hg.data(), sizeof(int) * (e->nbox + 1), hipMemcpyHostToDevice);
     if (er != hipSuccess) return MHD_ERR_HIP;
+    // boxes are packed by decreasing size: the ones with n <= tp are a tail
+    e->total = hg[e->nbox];
+    e->split = e->total;
+    e->nsmall = 0;
+    for (int q = 0; q < e->nbox; q++) {
+        const int n = e->n[e->order[q]];
+        if (n <= c.tp) {
+            e->split = hg[q];
+            e->nsmall = n;
+            break;
+        }
+    }
     e->shape = shape;
     e->grid = grid;
     drop_graph(e);
