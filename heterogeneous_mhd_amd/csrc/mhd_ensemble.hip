@@ -56,7 +56,7 @@ __device__ __forceinline__ int find_box(const int *pre, int nbox, int item)
     return lo;
 }
 
-template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+template <int BLOCK, int M, int TP, bool CFL, bool FAST, bool FIT = false>
 __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *Uout, const float *Uin,
                                                           const float *Bf, void *ws, int first,
                                                           int end)
@@ -68,13 +68,13 @@ __global__ __launch_bounds__(BLOCK) void ens_fluid_kernel(EnsDev E, float *Uout,
         const int n = E.n[box];
         __syncthreads();  // the previous item's last LDS reads are done
         // record index = item: each box's records are contiguous, in memory order
-        fluid_group<BLOCK, M, CFL, FAST, false, TP>((lds_f *)lds, (lds_f *)red, Uout + E.uoff[box], Uin + E.uoff[box],
+        fluid_group<BLOCK, M, CFL, FAST, false, TP, FIT>((lds_f *)lds, (lds_f *)red, Uout + E.uoff[box], Uin + E.uoff[box],
                                                     Bf + E.boff[box], n, n, n, TP,
                                                     item - E.gpre[box], E.dt[box], ws, item);
     }
 }
 
-template <int BLOCK, int M, int TP, bool FAST>
+template <int BLOCK, int M, int TP, bool FAST, bool FIT = false>
 __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout, const float *Bmain,
                                                            const float *U, const float *Bin, int first,
                                                            int end)
@@ -86,14 +86,14 @@ __global__ __launch_bounds__(BLOCK) void ens_advect_kernel(EnsDev E, float *Bout
         const long long bo = E.boff[box];
         __syncthreads();
         float keep[M][5];
-        advect_group<BLOCK, M, FAST, TP>((lds_f *)lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n,
+        advect_group<BLOCK, M, FAST, TP, FIT>((lds_f *)lds, Bout + bo, Bmain + bo, U + E.uoff[box], Bin + bo, n, n,
                                          n, TP, item - E.gpre[box], E.dt[box], keep);
     }
 }
 
 // advectbyzx + fluidx of one sweep in one pass (see mhd_fused.hip); u goes
 // to Uout, which must not be Uin
-template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+template <int BLOCK, int M, int TP, bool CFL, bool FAST, bool FIT = false>
 __global__ __launch_bounds__(BLOCK) void ens_fused_kernel(EnsDev E, float *Uout, float *Bout,
                                                           const float *Uin, const float *Bin, void *ws,
                                                           int first, int end)
@@ -107,10 +107,10 @@ __global__ __launch_bounds__(BLOCK) void ens_fused_kernel(EnsDev E, float *Uout,
         const int grp = item - E.gpre[box];
         __syncthreads();  // the previous item's last LDS reads are done
         float keep[M][5];
-        advect_group<BLOCK, M, FAST, TP>((lds_f *)lds, Bout + bo, Bin + bo, Uin + uo, Bin + bo, n, n, n, TP,
+        advect_group<BLOCK, M, FAST, TP, FIT>((lds_f *)lds, Bout + bo, Bin + bo, Uin + uo, Bin + bo, n, n, n, TP,
                                          grp, E.dt[box], keep);
         __syncthreads();
-        fluid_group<BLOCK, M, CFL, FAST, true, TP>((lds_f *)lds, (lds_f *)red, Uout + uo, Uin + uo, Bin + bo, n, n, n, TP,
+        fluid_group<BLOCK, M, CFL, FAST, true, TP, FIT>((lds_f *)lds, (lds_f *)red, Uout + uo, Uin + uo, Bin + bo, n, n, n, TP,
                                                    grp, E.dt[box], ws, item, keep);
     }
 }
@@ -189,48 +189,48 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kGridMult[] = {2, 4, 8, 16};  // persistent workgroups per CU
 constexpr int kNumGrids = sizeof(kGridMult) / sizeof(kGridMult[0]);
 
-template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+template <int BLOCK, int M, int TP, bool CFL, bool FAST, bool FIT>
 int launch_fluid_k(const EnsDev &E, float *Uo, const float *Ui, const float *B, size_t lds, void *ws,
                    int grid, hipStream_t s, int first, int end)
 {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, TP, CFL, FAST>,
+        (void)hipFuncSetAttribute((const void *)ens_fluid_kernel<BLOCK, M, TP, CFL, FAST, FIT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
-    hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, TP, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
+    hipLaunchKernelGGL((ens_fluid_kernel<BLOCK, M, TP, CFL, FAST, FIT>), dim3(grid), dim3(BLOCK), lds, s, E,
                        Uo, Ui, B, ws, first, end);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
-template <int BLOCK, int M, int TP, bool FAST>
+template <int BLOCK, int M, int TP, bool FAST, bool FIT>
 int launch_advect_k(const EnsDev &E, float *Bo, const float *Bm, const float *U, const float *Bi,
                     size_t lds, int grid, hipStream_t s, int first, int end)
 {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)ens_advect_kernel<BLOCK, M, TP, FAST>,
+        (void)hipFuncSetAttribute((const void *)ens_advect_kernel<BLOCK, M, TP, FAST, FIT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
-    hipLaunchKernelGGL((ens_advect_kernel<BLOCK, M, TP, FAST>), dim3(grid), dim3(BLOCK), lds, s, E, Bo,
+    hipLaunchKernelGGL((ens_advect_kernel<BLOCK, M, TP, FAST, FIT>), dim3(grid), dim3(BLOCK), lds, s, E, Bo,
                        Bm, U, Bi, first, end);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
-template <int BLOCK, int M, int TP, bool CFL, bool FAST>
+template <int BLOCK, int M, int TP, bool CFL, bool FAST, bool FIT>
 int launch_fused_k(const EnsDev &E, float *Uo, float *Bo, const float *Ui, const float *Bi, size_t lds,
                    void *ws, int grid, hipStream_t s, int first, int end)
 {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)ens_fused_kernel<BLOCK, M, TP, CFL, FAST>,
+        (void)hipFuncSetAttribute((const void *)ens_fused_kernel<BLOCK, M, TP, CFL, FAST, FIT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipGetLastError();
-    hipLaunchKernelGGL((ens_fused_kernel<BLOCK, M, TP, CFL, FAST>), dim3(grid), dim3(BLOCK), lds, s, E,
+    hipLaunchKernelGGL((ens_fused_kernel<BLOCK, M, TP, CFL, FAST, FIT>), dim3(grid), dim3(BLOCK), lds, s, E,
                        Uo, Bo, Ui, Bi, ws, first, end);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
 // One sweep kernel of one launch shape.  kind 0: fluid (Uo <- Ui, B0);
 // 1: advect (B0 -> Bo with b_main Bm, u U); 2: fused (Uo, Bo <- Ui, Bi).
-template <int BLOCK, int M, int TP>
+template <int BLOCK, int M, int TP, bool FIT = false>
 int launch_sweep(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo, const float *Ui,
                  const float *Bi, const float *Bm, int nmax, void *ws, int grid, hipStream_t s, int first,
                  int end)
@@ -238,23 +238,23 @@ int launch_sweep(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, floa
     constexpr int pb = BLOCK / TP;
     if (kind == 1) {
         const size_t lds = sizeof(float) * 2 * kLines * (size_t)nmax * pb;
-        return fast ? launch_advect_k<BLOCK, M, TP, true>(E, Bo, Bm, Ui, Bi, lds, grid, s, first, end)
-                    : launch_advect_k<BLOCK, M, TP, false>(E, Bo, Bm, Ui, Bi, lds, grid, s, first, end);
+        return fast ? launch_advect_k<BLOCK, M, TP, true, FIT>(E, Bo, Bm, Ui, Bi, lds, grid, s, first, end)
+                    : launch_advect_k<BLOCK, M, TP, false, FIT>(E, Bo, Bm, Ui, Bi, lds, grid, s, first, end);
     }
     if (kind == 0) {
         const size_t lds = sizeof(float) * 10 * (size_t)nmax * pb;
         if (cfl)
-            return fast ? launch_fluid_k<BLOCK, M, TP, true, true>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end)
-                        : launch_fluid_k<BLOCK, M, TP, true, false>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end);
-        return fast ? launch_fluid_k<BLOCK, M, TP, false, true>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end)
-                    : launch_fluid_k<BLOCK, M, TP, false, false>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end);
+            return fast ? launch_fluid_k<BLOCK, M, TP, true, true, FIT>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end)
+                        : launch_fluid_k<BLOCK, M, TP, true, false, FIT>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end);
+        return fast ? launch_fluid_k<BLOCK, M, TP, false, true, FIT>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end)
+                    : launch_fluid_k<BLOCK, M, TP, false, false, FIT>(E, Uo, Ui, Bi, lds, ws, grid, s, first, end);
     }
     const size_t lds = sizeof(float) * 10 * (size_t)nmax * pb;  // >= the advect's 2*kLines
     if (cfl)
-        return fast ? launch_fused_k<BLOCK, M, TP, true, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end)
-                    : launch_fused_k<BLOCK, M, TP, true, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end);
-    return fast ? launch_fused_k<BLOCK, M, TP, false, true>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end)
-                : launch_fused_k<BLOCK, M, TP, false, false>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end);
+        return fast ? launch_fused_k<BLOCK, M, TP, true, true, FIT>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end)
+                    : launch_fused_k<BLOCK, M, TP, true, false, FIT>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end);
+    return fast ? launch_fused_k<BLOCK, M, TP, false, true, FIT>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end)
+                : launch_fused_k<BLOCK, M, TP, false, false, FIT>(E, Uo, Bo, Ui, Bi, lds, ws, grid, s, first, end);
 }
 
 // The items of boxes that fit one cell per thread (n <= TP) form the tail of
@@ -263,28 +263,44 @@ int launch_sweep(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, floa
 // threads per pencil, so the items, groups and CFL records are the same and
 // no thread of a small box's pencil is idle.  split = the first such item
 // (total: none); nsmall = their largest side.
+// The head of the list, boxes of side exactly M*TP (items [0, nfit)), runs
+// the compile-time pencil-length form (immediate LDS offsets, no bounds tests).
 template <int BLOCK, int M, int TP>
 int sweep_split(int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo, const float *Ui,
-                const float *Bi, const float *Bm, int nmax, int nsmall, int split, int total, void *ws,
-                int grid, hipStream_t s)
+                const float *Bi, const float *Bm, int nmax, int nsmall, int nfit, int split, int nfit1,
+                int total, void *ws, int grid, hipStream_t s)
 {
-    if (M == 1 || split >= total)
-        return launch_sweep<BLOCK, M, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s, 0,
-                                          total);
     int rc = MHD_OK;
-    if (split > 0)
-        rc = launch_sweep<BLOCK, M, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s, 0, split);
-    if (rc == MHD_OK)
-        rc = launch_sweep<BLOCK, 1, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nsmall, ws, grid, s, split,
+    int from = 0;
+    if (nfit > 0) {
+        rc = launch_sweep<BLOCK, M, TP, true>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, M * TP, ws, grid, s,
+                                              0, nfit);
+        from = nfit;
+    }
+    if (rc != MHD_OK || from >= total) return rc;
+    if (M == 1 || split >= total)
+        return launch_sweep<BLOCK, M, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s, from,
+                                          total);
+    if (split > from)
+        rc = launch_sweep<BLOCK, M, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, ws, grid, s, from,
+                                        split);
+    from = split;
+    if (rc == MHD_OK && nfit1 > from) {  // small boxes of side exactly TP: the one-cell fixed form
+        rc = launch_sweep<BLOCK, 1, TP, true>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, TP, ws, grid, s, from,
+                                              nfit1);
+        from = nfit1;
+    }
+    if (rc == MHD_OK && from < total)
+        rc = launch_sweep<BLOCK, 1, TP>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nsmall, ws, grid, s, from,
                                         total);
     return rc;
 }
 
 int sweep(int shape, int kind, bool cfl, bool fast, const EnsDev &E, float *Uo, float *Bo,
-          const float *Ui, const float *Bi, const float *Bm, int nmax, int nsmall, int split, int total,
-          void *ws, int grid, hipStream_t s)
+          const float *Ui, const float *Bi, const float *Bm, int nmax, int nsmall, int nfit, int split,
+          int nfit1, int total, void *ws, int grid, hipStream_t s)
 {
-#define SW(B, M_, T) sweep_split<B, M_, T>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, nsmall, split, total, ws, grid, s)
+#define SW(B, M_, T) sweep_split<B, M_, T>(kind, cfl, fast, E, Uo, Bo, Ui, Bi, Bm, nmax, nsmall, nfit, split, nfit1, total, ws, grid, s)
     switch (shape) {  // the entries of kShapes
     case 0: return SW(256, 2, 64);
     case 1: return SW(128, 2, 64);
@@ -335,7 +351,7 @@ struct mhd_ensemble {
     std::vector<long long> uoff, boff;  // per box (caller's order)
     size_t ufloats, bfloats;
     int shape, grid;                    // current launch shape and persistent grid size
-    int split, nsmall, total;           // see sweep_split (set with the shape)
+    int split, nsmall, nfit, nfit1, total;  // see sweep_split (set with the shape)
     bool fast;                          // FP32 tolerance mode of the sweeps
     // device tables
     void *tab;
@@ -366,7 +382,7 @@ int ens_step(mhd_ensemble *e)
     const hipStream_t s = e->s;
     float *U0 = e->U0, *B0 = e->B0, *U1 = e->U1, *B1 = e->B1;
     const int sh = e->shape, g = e->grid, nm = e->nmax;
-    const int ns = e->nsmall, sp = e->split, tot = e->total;
+    const int ns = e->nsmall, nf = e->nfit, sp = e->split, nf1 = e->nfit1, tot = e->total;
     const bool f = e->fast;
     int rc = MHD_OK;
 #define RUN(x)                      \
@@ -378,19 +394,19 @@ int ens_step(mhd_ensemble *e)
     RUN(ens_error(hipGetLastError()));
     // the order of C/mhd.c:71-117; the second half's advectbyzx + fluidx pairs
     // are fused, and u ping-pongs so that the step ends in (U0, B0)
-    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U0, B0, nullptr, nm, ns, sp, tot, nullptr, g, s));   // :71
-    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, ns, sp, tot, nullptr, g, s));        // :74
+    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U0, B0, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));   // :71
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, ns, nf, sp, nf1, tot, nullptr, g, s));        // :74
     RUN(launch_transpose(E, U0, B0, U1, B1, false, g, s));                             // :78
-    RUN(sweep(sh, 0, false, f, E, U0, nullptr, U0, B0, nullptr, nm, ns, sp, tot, nullptr, g, s));   // :80
-    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U0, B0, B0, nm, ns, sp, tot, nullptr, g, s));        // :84
+    RUN(sweep(sh, 0, false, f, E, U0, nullptr, U0, B0, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));   // :80
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U0, B0, B0, nm, ns, nf, sp, nf1, tot, nullptr, g, s));        // :84
     RUN(launch_transpose(E, U1, B0, U0, B1, true, g, s));                              // :88
-    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U1, B0, nullptr, nm, ns, sp, tot, nullptr, g, s));   // :91
-    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, ns, sp, tot, nullptr, g, s));        // :93
-    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, ns, sp, tot, nullptr, g, s));        // :95-97
+    RUN(sweep(sh, 0, false, f, E, U1, nullptr, U1, B0, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));   // :91
+    RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, ns, nf, sp, nf1, tot, nullptr, g, s));        // :93
+    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));        // :95-97
     RUN(launch_transpose(E, U1, B1, U0, B0, true, g, s));                              // :102
-    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, ns, sp, tot, nullptr, g, s));        // :105-107
+    RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));        // :105-107
     RUN(launch_transpose(E, U1, B1, U0, B0, false, g, s));                             // :112
-    RUN(sweep(sh, 2, true, f, E, U0, B0, U1, B1, nullptr, nm, ns, sp, tot, e->ws, g, s));           // :115-117 + next calcfl
+    RUN(sweep(sh, 2, true, f, E, U0, B0, U1, B1, nullptr, nm, ns, nf, sp, nf1, tot, e->ws, g, s));           // :115-117 + next calcfl
     if (rc == MHD_OK) {
         hipLaunchKernelGGL(ens_cfl_fold_kernel, dim3(e->nbox), dim3(kFoldBlock), 0, s, E, e->ws, e->cfl);
         rc = ens_error(hipGetLastError());
@@ -566,6 +582,8 @@ int mhd_ensemble_set_config(mhd_ensemble *e, int shape, int grid)
     e->total = hg[e->nbox];
     e->split = e->total;
     e->nsmall = 0;
+    e->nfit = 0;  // the head boxes of side exactly m*tp
+    for (int q = 0; q < e->nbox && e->n[e->order[q]] == c.m * c.tp; q++) e->nfit = hg[q + 1];
     for (int q = 0; q < e->nbox; q++) {
         const int n = e->n[e->order[q]];
         if (n <= c.tp) {
@@ -574,6 +592,9 @@ int mhd_ensemble_set_config(mhd_ensemble *e, int shape, int grid)
             break;
         }
     }
+    e->nfit1 = e->split;  // then the small boxes of side exactly tp
+    for (int q = 0; q < e->nbox; q++)
+        if (hg[q] >= e->split && e->n[e->order[q]] == c.tp) e->nfit1 = hg[q + 1];
     e->shape = shape;
     e->grid = grid;
     drop_graph(e);
