@@ -82,3 +82,33 @@ def test_ensemble_tune_keeps_state_and_results(gpu):
         u, b = ens.box_state(i)
         assert bits_equal(u, ref[i][0]) and bits_equal(b, ref[i][1])
         assert np.float32(t[i]) == np.float32(ref[i][2])
+
+
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_ensemble_size_ranges_match_single_box_runs(gpu, precision):
+    """With the default shape (2 cells x 64 threads per pencil) a sweep runs
+    in ranges: boxes of side 128 on the compile-time pencil length, 96 on the
+    generic two-cell form, 64 on the one-cell fixed form and 40, 32 on the
+    one-cell generic form.  Every box still equals its single-box run."""
+    import torch
+    from heterogeneous_mhd_amd import solver
+    sides = [64, 128, 32, 96, 40]
+    _, specs = solver.ensemble_specs(len(sides), seed=11)
+    print("ranges", sides, precision, flush=True)
+    ref = []
+    for n, sp in zip(sides, specs):
+        box = solver.Box(n, n, n, cfl=0.7, precision=precision)
+        box.init(sp)
+        box.run(2)
+        ref.append((*box.host_state(), *box.time()))
+        box.close()
+    ens = solver.Ensemble(sides, cfl=0.7, precision=precision)
+    ens.init(specs)
+    ens.run(2, graph=True)
+    t, dt = ens.times()
+    for i in range(len(sides)):
+        u, b = ens.box_state(i)
+        assert bits_equal(u, ref[i][0]) and bits_equal(b, ref[i][1]), f"box {i} ({sides[i]}^3)"
+        assert np.float32(t[i]) == np.float32(ref[i][2]) and np.float32(dt[i]) == np.float32(ref[i][3])
+    ens.close()
+    torch.cuda.empty_cache()
