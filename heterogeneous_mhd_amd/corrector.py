@@ -61,7 +61,9 @@ class Buffers:
             raise MHDError("the corrector needs nx a multiple of 64 and nx*ny*nz of 256")
         f32, i16 = torch.float32, torch.int16  # int16 holds bf16 bits
         self.x0 = torch.empty(v * CH, dtype=i16, device=device)
-        self.z = [torch.empty(v * CH, dtype=f32, device=device) for _ in range(LAYERS)]
+        # hidden pre-activations are kept only for a backward pass
+        self.z = [torch.empty(v * CH, dtype=f32, device=device) if train or li == LAYERS - 1 else None
+                  for li in range(LAYERS)]
         self.a = [torch.empty(v * CH, dtype=i16, device=device) for _ in range(LAYERS - 1)]
         if train:
             self.d = [torch.empty(v * CH, dtype=i16, device=device) for _ in range(2)]
@@ -142,7 +144,8 @@ class CorrectorNet:
         for li in range(LAYERS):
             last = li == LAYERS - 1
             check(L.mhd_corr_conv(_p(a), _p(self.wf[li]), _p(self.bias(li)), nx, ny, nz,
-                                  1 if last else 0, _p(buf.z[li]), None if last else _p(buf.a[li]),
+                                  1 if last else 0, None if buf.z[li] is None else _p(buf.z[li]),
+                                  None if last else _p(buf.a[li]),
                                   None, s), "corr_conv")
             if not last:
                 a = buf.a[li]

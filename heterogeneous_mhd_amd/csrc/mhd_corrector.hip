@@ -17,11 +17,13 @@
 // of the 27 taps is one K-step of 32 input channels: the A operand (16
 // voxels x 32 channels) is one 16-byte load per lane straight from HBM/L2,
 // the B operand (32 channels x 16 outputs) one 16-byte LDS read per lane from
-// the weights staged once per workgroup; v_mfma_f32_16x16x32_bf16 does
-// the product, 8 per tap (4 voxel blocks x 2 output blocks).  The input
+// the weights staged once per 8-wavefront workgroup; v_mfma_f32_16x16x32_bf16
+// does the product, 8 per tap (4 voxel blocks x 2 output blocks).  The A
+// loads run two taps ahead of the MFMAs from a wave-uniform row base.  The
+// input
 // gradient is the same convolution with taps mirrored and the weight matrix
 // transposed (mhd_corr_weight_prep writes both forms).  Epilogues: bias +
-// GELU (storing the FP32 pre-activation for the backward pass), bias only
+// GELU (storing the FP32 pre-activation when a backward pass follows), bias only
 // (output layer), or multiplication by GELU'(z) of the previous layer (the
 // backward pass through the activation).
 //
@@ -57,14 +59,44 @@ __device__ __forceinline__ u16 f2bf(float x)
     u += 0x7fffu + ((u >> 16) & 1u);                                        // nearest even
     return (u16)(u >> 16);
 }
+// the same rounding by the gfx950 conversion instruction (finite inputs)
+__device__ __forceinline__ u16 bf16_rne(float x) { return __builtin_bit_cast(u16, (__bf16)x); }
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
 __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// GELU in its erfc form, 0.5 z erfc(-z/sqrt2): no cancellation for z < 0
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * erfcf(-z * 0.70710678118654752f); }
+// Phi(z) = 0.5 erfc(-z/sqrt2), the standard normal CDF, in the form of
+// Numerical Recipes' erfcc: erfc(x) = t exp(-x^2 + P(t)), t = 1/(1 + x/2),
+// x >= 0, with our own degree-9 fit of P = log(exp(x^2) erfc(x) / t) on
+// t in [0, 1] (max |error in P| 1.0e-7, so erfc to ~1e-7 relative before
+// FP32 rounding of -x^2).  About 15 operations and one exp against ~60 for
+// erfcf; the activations it feeds are rounded to bf16.
+constexpr float kPhiP[10] = {
+    -1.265512220e+00f, 1.000022661e+00f, 3.741248111e-01f,
+    9.639016016e-02f, -1.839756770e-01f, 2.713769287e-01f,
+    -1.121139396e+00f, 1.473290611e+00f, -8.133419797e-01f,
+    1.687640158e-01f};
+
+struct PhiExp {
+    float phi;  // Phi(z)
+    float g;    // exp(-z^2/2)
+};
+template <bool WANT_G>
+__device__ __forceinline__ PhiExp phi_of(float z)
+{
+    const float x = fabsf(z) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f, x, 1.0f));
+    float p = kPhiP[9];
+#pragma unroll
+    for (int q = 8; q >= 0; q--) p = __builtin_fmaf(p, t, kPhiP[q]);
+    const float x2 = x * x;
+    const float half_erfc = 0.5f * t * __expf(p - x2);
+    return PhiExp{z < 0.0f ? half_erfc : 1.0f - half_erfc, WANT_G ? __expf(-x2) : 0.0f};
+}
+__device__ __forceinline__ float gelu(float z) { return z * phi_of<false>(z).phi; }
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    return 0.5f * erfcf(-z * 0.70710678118654752f) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+    const PhiExp r = phi_of<true>(z);
+    return r.phi + z * r.g * 0.39894228040143268f;  // Phi + z phi(z)
 }
 
 enum { kModeHidden = 0, kModeOutput = 1, kModeGrad = 2 };
@@ -75,23 +107,51 @@ __device__ __forceinline__ float ylim(float y, float lim)
     return lim > 0.0f ? fminf(fmaxf(y, -lim), lim) : y;
 }
 
+constexpr int kConvWaves = 8;    // wavefronts per forward-kernel workgroup
+constexpr int kConvPF = 2;       // taps of A operands kept in flight ahead of the MFMAs
+
 template <int MODE>
-__global__ __launch_bounds__(256) void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
-                                                   const float *__restrict__ bias, int nx, int ny,
-                                                   int nz, float *__restrict__ zout,
-                                                   u16 *__restrict__ aout,
-                                                   const float *__restrict__ zprev)
+__global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu(4)))
+void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
+                 const float *__restrict__ bias, int nx, int ny, int nz,
+                 float *__restrict__ zout, u16 *__restrict__ aout,
+                 const float *__restrict__ zprev)
 {
     __shared__ __attribute__((aligned(16))) u16 wl[kWElems];
-    for (int f = threadIdx.x; f < kWElems / 8; f += 256)
+    for (int f = threadIdx.x; f < kWElems / 8; f += 64 * kConvWaves)
         reinterpret_cast<uint4 *>(wl)[f] = reinterpret_cast<const uint4 *>(W)[f];
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int lm = lane & 15, lg = lane >> 4;
-    const long v0 = ((long)blockIdx.x * 4 + wave) * 64;  // this wavefront's 64 voxels
-    const int i0 = (int)(v0 % nx);
+    const long v0 = ((long)blockIdx.x * kConvWaves + wave) * 64;  // this wavefront's 64 voxels
+    if (v0 >= (long)nx * ny * nz) return;                         // tail workgroup: no work left
     const long row = v0 / nx;
-    const int j = (int)(row % ny), k = (int)(row / ny);
+    const int i0 = __builtin_amdgcn_readfirstlane((int)(v0 - row * nx));
+    const int j = __builtin_amdgcn_readfirstlane((int)(row % ny));
+    const int k = __builtin_amdgcn_readfirstlane((int)(row / ny));
+
+    // A operand of tap t: 16 voxels x 32 input channels per voxel block mb.
+    // The neighbour row is a wave-uniform base and the lane offsets are
+    // computed once, so a tap costs no vector address arithmetic.  A shift
+    // of one voxel is +-64 bytes (an immediate offset) except where block 0
+    // (shift -1) or block 3 (shift +1) wraps around the periodic box.
+    unsigned ac[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) ac[mb] = 2u * (unsigned)((i0 + 16 * mb + lm) * kC + 8 * lg);
+    const unsigned aw = 2u * (unsigned)(wrap(i0 + lm - 1, nx) * kC + 8 * lg);
+    const unsigned ae = 2u * (unsigned)(wrap(i0 + 48 + lm + 1, nx) * kC + 8 * lg);
+    auto load_tap = [&](bf16x8 (&av)[4], int t) {
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        const char *rb = reinterpret_cast<const char *>(
+            X + ((size_t)wrap(k + dz, nz) * ny + wrap(j + dy, ny)) * nx * kC);
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            const char *p = dx < 0 && mb == 0 ? rb + aw
+                          : dx > 0 && mb == 3 ? rb + ae
+                          : rb + 2 * kC * dx + ac[mb];
+            av[mb] = *reinterpret_cast<const bf16x8 *>(p);
+        }
+    };
 
     f32x4 acc[4][2];
 #pragma unroll
@@ -99,26 +159,36 @@ __global__ __launch_bounds__(256) void conv_kernel(const u16 *__restrict__ X, co
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
+    // ring of kConvPF + 1 taps: the loads of tap t + kConvPF are issued before
+    // the MFMAs of tap t wait for theirs (fully unrolled, so the ring slots are
+    // static registers)
+    bf16x8 av[kConvPF + 1][4];
+#pragma unroll
+    for (int t = 0; t < kConvPF; t++) load_tap(av[t], t);
+#pragma unroll
     for (int t = 0; t < kTaps; t++) {
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        const long rb = ((long)wrap(k + dz, nz) * ny + wrap(j + dy, ny)) * nx;
+        if (t + kConvPF < kTaps) load_tap(av[(t + kConvPF) % (kConvPF + 1)], t + kConvPF);
         bf16x8 bw[2];
 #pragma unroll
         for (int nb = 0; nb < 2; nb++)
             bw[nb] = *reinterpret_cast<const bf16x8 *>(wl + (t * kC + 16 * nb + lm) * kC + 8 * lg);
-        bf16x8 av[4];
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++) {
-            const int ii = wrap(i0 + 16 * mb + lm + dx, nx);
-            av[mb] = *reinterpret_cast<const bf16x8 *>(X + (rb + ii) * kC + 8 * lg);
-        }
+        const int slot = t % (kConvPF + 1);
 #pragma unroll
         for (int mb = 0; mb < 4; mb++)
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[mb], bw[nb], acc[mb][nb], 0, 0, 0);
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[slot][mb], bw[nb], acc[mb][nb],
+                                                                      0, 0, 0);
+        // keep the scheduler from hoisting later taps' loads past the ring
+        // (they would need registers beyond the 4-waves-per-SIMD budget)
+        __builtin_amdgcn_sched_barrier(0);
     }
-    // lane holds rows (voxels) 16*mb + 4*lg + r, column (channel) 16*nb + lm
+    // lane holds rows (voxels) 16*mb + 4*lg + r, column (channel) 16*nb + lm:
+    // element offset (v - v0)*32 + o from the wave's uniform base v0*32
+    const size_t vb = (size_t)v0 * kC;
+    float *zw = zout ? zout + vb : nullptr;
+    u16 *aw16 = aout ? aout + vb : nullptr;
+    const float *zp = zprev ? zprev + vb : nullptr;
 #pragma unroll
     for (int nb = 0; nb < 2; nb++) {
         const int o = 16 * nb + lm;
@@ -127,16 +197,16 @@ __global__ __launch_bounds__(256) void conv_kernel(const u16 *__restrict__ X, co
         for (int mb = 0; mb < 4; mb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const long v = v0 + 16 * mb + 4 * lg + r;
+                const unsigned e = (unsigned)((16 * mb + 4 * lg + r) * kC + o);
                 const float x = acc[mb][nb][r];
                 if (MODE == kModeHidden) {
                     const float z = x + bo;
-                    zout[v * kC + o] = z;
-                    aout[v * kC + o] = f2bf(gelu(z));
+                    if (zw) zw[e] = z;  // kept for a backward pass only
+                    aw16[e] = bf16_rne(gelu(z));
                 } else if (MODE == kModeOutput) {
-                    zout[v * kC + o] = x + bo;
+                    zw[e] = x + bo;
                 } else {
-                    aout[v * kC + o] = f2bf(x * gelu_grad(zprev[v * kC + o]));
+                    aw16[e] = bf16_rne(x * gelu_grad(zp[e]));
                 }
             }
     }
@@ -566,22 +636,22 @@ int mhd_corr_conv(const uint16_t *x, const uint16_t *w, const float *bias, int n
 {
     if (!x || !w || !conv_shape_ok(nx, ny, nz)) return MHD_ERR_ARG;
     const hipStream_t s = (hipStream_t)stream;
-    const unsigned grid = (unsigned)((long)nx * ny * nz / 256);
+    const unsigned grid = (unsigned)(((long)nx * ny * nz / 64 + kConvWaves - 1) / kConvWaves);
     (void)hipGetLastError();
     switch (mode) {
     case kModeHidden:
-        if (!bias || !z_out || !a_out) return MHD_ERR_ARG;
-        hipLaunchKernelGGL(conv_kernel<kModeHidden>, dim3(grid), dim3(256), 0, s, x, w, bias, nx, ny,
+        if (!bias || !a_out) return MHD_ERR_ARG;
+        hipLaunchKernelGGL(conv_kernel<kModeHidden>, dim3(grid), dim3(64 * kConvWaves), 0, s, x, w, bias, nx, ny,
                            nz, z_out, a_out, nullptr);
         break;
     case kModeOutput:
         if (!bias || !z_out) return MHD_ERR_ARG;
-        hipLaunchKernelGGL(conv_kernel<kModeOutput>, dim3(grid), dim3(256), 0, s, x, w, bias, nx, ny,
+        hipLaunchKernelGGL(conv_kernel<kModeOutput>, dim3(grid), dim3(64 * kConvWaves), 0, s, x, w, bias, nx, ny,
                            nz, z_out, nullptr, nullptr);
         break;
     case kModeGrad:
         if (!a_out || !z_prev) return MHD_ERR_ARG;
-        hipLaunchKernelGGL(conv_kernel<kModeGrad>, dim3(grid), dim3(256), 0, s, x, w, nullptr, nx, ny,
+        hipLaunchKernelGGL(conv_kernel<kModeGrad>, dim3(grid), dim3(64 * kConvWaves), 0, s, x, w, nullptr, nx, ny,
                            nz, nullptr, a_out, z_prev);
         break;
     default: return MHD_ERR_ARG;

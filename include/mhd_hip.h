@@ -272,6 +272,7 @@ int mhd_write_check_dumps(const char *dir, const float *u, const float *b, int n
 /* floats per layer: 27*32*32 weights followed by 32 biases */
 int mhd_corr_num_params(void);
 /* mode 0: z_out = conv(x) + bias (FP32 [V][32]), a_out = bf16(GELU(z));
+ *         z_out may be NULL when no backward pass follows (not stored);
  * mode 1: z_out = conv(x) + bias;
  * mode 2: a_out = bf16(conv(x) * GELU'(z_prev)) -- with the input-gradient
  *         weights of mhd_corr_weight_prep, the backward pass of a layer. */
