@@ -13,7 +13,8 @@
 // periodic, as the box is.
 //
 // Forward and input-gradient kernel (implicit GEMM).  A wavefront owns 64
-// consecutive voxels of one x-row (M) and all 32 output channels (N).  Each
+// consecutive voxels of one x-row (M; two adjacent y-rows when ny is even,
+// sharing the A loads) and all 32 output channels (N).  Each
 // of the 27 taps is one K-step of 32 input channels: the A operand (16
 // voxels x 32 channels) is one 16-byte load per lane straight from HBM/L2,
 // the B operand (32 channels x 16 outputs) one 16-byte LDS read per lane from
@@ -108,9 +109,15 @@ __device__ __forceinline__ float ylim(float y, float lim)
 }
 
 constexpr int kConvWaves = 8;    // wavefronts per forward-kernel workgroup
-constexpr int kConvPF = 2;       // taps of A operands kept in flight ahead of the MFMAs
 
-template <int MODE>
+// A wavefront owns R adjacent y-rows (one z plane) of one 64-voxel x-chunk.
+// It walks the 3 (R + 2) (3 dz x R + 2 input rows) neighbour rows once per
+// x shift; each loaded A operand feeds every output row it is a tap of (up
+// to 3), so A traffic per output row falls from 27 to 9 (R + 2) / R loads.
+// Each output row still accumulates its taps in the order t = 0..26, so the
+// result does not depend on R.  PF steps of A loads are in flight ahead of
+// the MFMAs.
+template <int MODE, int R, int PF>
 __global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
                  const float *__restrict__ bias, int nx, int ny, int nz,
@@ -123,27 +130,30 @@ void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int lm = lane & 15, lg = lane >> 4;
-    const long v0 = ((long)blockIdx.x * kConvWaves + wave) * 64;  // this wavefront's 64 voxels
-    if (v0 >= (long)nx * ny * nz) return;                         // tail workgroup: no work left
-    const long row = v0 / nx;
-    const int i0 = __builtin_amdgcn_readfirstlane((int)(v0 - row * nx));
-    const int j = __builtin_amdgcn_readfirstlane((int)(row % ny));
-    const int k = __builtin_amdgcn_readfirstlane((int)(row / ny));
+    const int cx = nx / 64, nyr = ny / R;
+    const long g = (long)blockIdx.x * kConvWaves + wave;  // this wavefront's R x 64 voxels
+    if (g >= (long)cx * nyr * nz) return;                 // tail workgroup: no work left
+    const long pr = g / cx;
+    const int i0 = __builtin_amdgcn_readfirstlane((int)(g - pr * cx) * 64);
+    const int j = __builtin_amdgcn_readfirstlane((int)(pr % nyr) * R);
+    const int k = __builtin_amdgcn_readfirstlane((int)(pr / nyr));
 
-    // A operand of tap t: 16 voxels x 32 input channels per voxel block mb.
-    // The neighbour row is a wave-uniform base and the lane offsets are
-    // computed once, so a tap costs no vector address arithmetic.  A shift
-    // of one voxel is +-64 bytes (an immediate offset) except where block 0
-    // (shift -1) or block 3 (shift +1) wraps around the periodic box.
+    // A operand: 16 voxels x 32 input channels per voxel block mb.  The
+    // neighbour row is a wave-uniform base and the lane offsets are computed
+    // once, so a step costs no vector address arithmetic.  A shift of one
+    // voxel is +-64 bytes (an immediate offset) except where block 0 (shift
+    // -1) or block 3 (shift +1) wraps around the periodic box.
     unsigned ac[4];
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) ac[mb] = 2u * (unsigned)((i0 + 16 * mb + lm) * kC + 8 * lg);
     const unsigned aw = 2u * (unsigned)(wrap(i0 + lm - 1, nx) * kC + 8 * lg);
     const unsigned ae = 2u * (unsigned)(wrap(i0 + 48 + lm + 1, nx) * kC + 8 * lg);
-    auto load_tap = [&](bf16x8 (&av)[4], int t) {
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    constexpr int NJ = R + 2, NS = 9 * NJ;  // input rows per plane, load steps
+    // step s: plane dz = s / (3 NJ) - 1, input row j - 1 + (s / 3) % NJ, shift s % 3 - 1
+    auto load_step = [&](bf16x8 (&av)[4], int s) {
+        const int dz = s / (3 * NJ) - 1, jj = (s / 3) % NJ, dx = s % 3 - 1;
         const char *rb = reinterpret_cast<const char *>(
-            X + ((size_t)wrap(k + dz, nz) * ny + wrap(j + dy, ny)) * nx * kC);
+            X + ((size_t)wrap(k + dz, nz) * ny + wrap(j - 1 + jj, ny)) * nx * kC);
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) {
             const char *p = dx < 0 && mb == 0 ? rb + aw
@@ -153,63 +163,93 @@ void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
         }
     };
 
-    f32x4 acc[4][2];
+    f32x4 acc[R][4][2];
 #pragma unroll
-    for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-        for (int nb = 0; nb < 2; nb++) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-
-    // ring of kConvPF + 1 taps: the loads of tap t + kConvPF are issued before
-    // the MFMAs of tap t wait for theirs (fully unrolled, so the ring slots are
-    // static registers)
-    bf16x8 av[kConvPF + 1][4];
-#pragma unroll
-    for (int t = 0; t < kConvPF; t++) load_tap(av[t], t);
-#pragma unroll
-    for (int t = 0; t < kTaps; t++) {
-        if (t + kConvPF < kTaps) load_tap(av[(t + kConvPF) % (kConvPF + 1)], t + kConvPF);
-        bf16x8 bw[2];
-#pragma unroll
-        for (int nb = 0; nb < 2; nb++)
-            bw[nb] = *reinterpret_cast<const bf16x8 *>(wl + (t * kC + 16 * nb + lm) * kC + 8 * lg);
-        const int slot = t % (kConvPF + 1);
+    for (int r = 0; r < R; r++)
 #pragma unroll
         for (int mb = 0; mb < 4; mb++)
 #pragma unroll
+            for (int nb = 0; nb < 2; nb++) acc[r][mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    // ring of PF + 1 steps, fully unrolled so the slots are static registers
+    bf16x8 av[PF + 1][4];
+#pragma unroll
+    for (int s = 0; s < PF; s++) load_step(av[s], s);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        if (s + PF < NS) load_step(av[(s + PF) % (PF + 1)], s + PF);
+        const int slot = s % (PF + 1);
+        const int dz = s / (3 * NJ), jj = (s / 3) % NJ, dx = s % 3;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int dy = jj - r;  // tap row index 0..2 of output row j + r
+            if (dy < 0 || dy > 2) continue;
+            const int t = dz * 9 + dy * 3 + dx;
+            bf16x8 bw[2];
+#pragma unroll
             for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[slot][mb], bw[nb], acc[mb][nb],
-                                                                      0, 0, 0);
-        // keep the scheduler from hoisting later taps' loads past the ring
+                bw[nb] = *reinterpret_cast<const bf16x8 *>(wl + (t * kC + 16 * nb + lm) * kC + 8 * lg);
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++)
+                    acc[r][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[slot][mb], bw[nb],
+                                                                             acc[r][mb][nb], 0, 0, 0);
+        }
+        // keep the scheduler from hoisting later steps' loads past the ring
         // (they would need registers beyond the 4-waves-per-SIMD budget)
         __builtin_amdgcn_sched_barrier(0);
     }
-    // lane holds rows (voxels) 16*mb + 4*lg + r, column (channel) 16*nb + lm:
-    // element offset (v - v0)*32 + o from the wave's uniform base v0*32
-    const size_t vb = (size_t)v0 * kC;
-    float *zw = zout ? zout + vb : nullptr;
-    u16 *aw16 = aout ? aout + vb : nullptr;
-    const float *zp = zprev ? zprev + vb : nullptr;
+    // lane holds rows (voxels) 16*mb + 4*lg + q, column (channel) 16*nb + lm:
+    // element offset (v - v0)*32 + o from output row r's uniform base v0*32
 #pragma unroll
-    for (int nb = 0; nb < 2; nb++) {
-        const int o = 16 * nb + lm;
-        const float bo = MODE == kModeGrad ? 0.0f : bias[o];
+    for (int r = 0; r < R; r++) {
+        const size_t vb = (((size_t)k * ny + j + r) * nx + i0) * kC;
+        float *zw = zout ? zout + vb : nullptr;
+        u16 *aw16 = aout ? aout + vb : nullptr;
+        const float *zp = zprev ? zprev + vb : nullptr;
 #pragma unroll
-        for (int mb = 0; mb < 4; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            const int o = 16 * nb + lm;
+            const float bo = MODE == kModeGrad ? 0.0f : bias[o];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const unsigned e = (unsigned)((16 * mb + 4 * lg + r) * kC + o);
-                const float x = acc[mb][nb][r];
-                if (MODE == kModeHidden) {
-                    const float z = x + bo;
-                    if (zw) zw[e] = z;  // kept for a backward pass only
-                    aw16[e] = bf16_rne(gelu(z));
-                } else if (MODE == kModeOutput) {
-                    zw[e] = x + bo;
-                } else {
-                    aw16[e] = bf16_rne(x * gelu_grad(zp[e]));
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const unsigned e = (unsigned)((16 * mb + 4 * lg + q) * kC + o);
+                    const float x = acc[r][mb][nb][q];
+                    if (MODE == kModeHidden) {
+                        const float z = x + bo;
+                        if (zw) zw[e] = z;  // kept for a backward pass only
+                        aw16[e] = bf16_rne(gelu(z));
+                    } else if (MODE == kModeOutput) {
+                        zw[e] = x + bo;
+                    } else {
+                        aw16[e] = bf16_rne(x * gelu_grad(zp[e]));
+                    }
                 }
-            }
+        }
     }
+}
+
+constexpr int kConvPF1 = 2;  // A load steps in flight, one row per wavefront
+constexpr int kConvPF2 = 1;  // the same, two rows per wavefront (64 more accumulator registers)
+
+template <int R, int PF>
+void launch_conv(int mode, const u16 *x, const u16 *w, const float *bias, int nx, int ny, int nz,
+                 float *z_out, u16 *a_out, const float *z_prev, hipStream_t s)
+{
+    const long waves = (long)nx * ny * nz / (64 * R);
+    const dim3 grid((unsigned)((waves + kConvWaves - 1) / kConvWaves)), block(64 * kConvWaves);
+    if (mode == kModeHidden)
+        hipLaunchKernelGGL((conv_kernel<kModeHidden, R, PF>), grid, block, 0, s, x, w, bias, nx, ny, nz,
+                           z_out, a_out, nullptr);
+    else if (mode == kModeOutput)
+        hipLaunchKernelGGL((conv_kernel<kModeOutput, R, PF>), grid, block, 0, s, x, w, bias, nx, ny, nz,
+                           z_out, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((conv_kernel<kModeGrad, R, PF>), grid, block, 0, s, x, w, nullptr, nx, ny, nz,
+                           nullptr, a_out, z_prev);
 }
 
 // Weight gradient, see the file header.  ws[g] (kParams floats per
@@ -635,27 +675,16 @@ int mhd_corr_conv(const uint16_t *x, const uint16_t *w, const float *bias, int n
                   int mode, float *z_out, uint16_t *a_out, const float *z_prev, mhd_stream_t stream)
 {
     if (!x || !w || !conv_shape_ok(nx, ny, nz)) return MHD_ERR_ARG;
-    const hipStream_t s = (hipStream_t)stream;
-    const unsigned grid = (unsigned)(((long)nx * ny * nz / 64 + kConvWaves - 1) / kConvWaves);
-    (void)hipGetLastError();
     switch (mode) {
-    case kModeHidden:
-        if (!bias || !a_out) return MHD_ERR_ARG;
-        hipLaunchKernelGGL(conv_kernel<kModeHidden>, dim3(grid), dim3(64 * kConvWaves), 0, s, x, w, bias, nx, ny,
-                           nz, z_out, a_out, nullptr);
-        break;
-    case kModeOutput:
-        if (!bias || !z_out) return MHD_ERR_ARG;
-        hipLaunchKernelGGL(conv_kernel<kModeOutput>, dim3(grid), dim3(64 * kConvWaves), 0, s, x, w, bias, nx, ny,
-                           nz, z_out, nullptr, nullptr);
-        break;
-    case kModeGrad:
-        if (!a_out || !z_prev) return MHD_ERR_ARG;
-        hipLaunchKernelGGL(conv_kernel<kModeGrad>, dim3(grid), dim3(64 * kConvWaves), 0, s, x, w, nullptr, nx, ny,
-                           nz, nullptr, a_out, z_prev);
-        break;
+    case kModeHidden: if (!bias || !a_out) return MHD_ERR_ARG; break;
+    case kModeOutput: if (!bias || !z_out) return MHD_ERR_ARG; break;
+    case kModeGrad: if (!a_out || !z_prev) return MHD_ERR_ARG; break;
     default: return MHD_ERR_ARG;
     }
+    (void)hipGetLastError();
+    const hipStream_t s = (hipStream_t)stream;
+    if (ny % 2 == 0) launch_conv<2, kConvPF2>(mode, x, w, bias, nx, ny, nz, z_out, a_out, z_prev, s);
+    else launch_conv<1, kConvPF1>(mode, x, w, bias, nx, ny, nz, z_out, a_out, z_prev, s);
     return done();
 }
 
