@@ -384,16 +384,38 @@ class CorrectedBox:
         self.box = solver.Box(n, n, n, cfl=cfl)
         self.buf = Buffers(n, n, n, net.device, train=False)
         self.stats = torch.zeros(3, dtype=torch.int32, device=net.device)
+        self._graph = None
 
-    def step(self, nsteps: int = 1):
+    def _correct(self):
         bx = self.box
-        for _ in range(nsteps):
-            bx.run(1, graph=True)
-            with torch.cuda.stream(bx.stream):
-                y = self.net.forward(self.buf, bx.u, bx.b)
-                self.net.apply(self.buf, bx.u, bx.b, y, self.etol, self.stats, self.scale,
-                               self.limit)
-            bx.state_changed()
+        with torch.cuda.stream(bx.stream):
+            y = self.net.forward(self.buf, bx.u, bx.b)
+            self.net.apply(self.buf, bx.u, bx.b, y, self.etol, self.stats, self.scale, self.limit)
+
+    def step(self, nsteps: int = 1, graph: bool = False):
+        """nsteps corrected steps.  graph=True replays one HIP graph per step
+        (the CFL of the corrected state, the step, the network and the
+        checked correction: one launch instead of ~12 at these small sizes);
+        the result is bitwise that of the eager form."""
+        bx = self.box
+        if not graph:
+            for _ in range(nsteps):
+                bx.run(1, graph=True)
+                self._correct()
+                bx.state_changed()
+            return
+        if self._graph is None:
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=bx.stream):
+                bx.state_changed()       # the step below starts with calcfl of this state
+                bx.run(1, graph=False)   # enqueued on bx.stream, so captured
+                self._correct()
+            self._graph = g
+        with torch.cuda.stream(bx.stream):
+            for _ in range(nsteps):
+                self._graph.replay()
+        bx.state_changed()
 
     def fallbacks(self):
         s = self.stats.cpu().tolist()
@@ -415,9 +437,22 @@ def rel_error(u, b, uref, bref, n):
 def _timed(fn):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fn()
+    out = fn()
     torch.cuda.synchronize()
-    return time.perf_counter() - t0
+    return time.perf_counter() - t0, out
+
+
+def _device_step_s(stream, run, k: int = 40, warm: int = 3) -> float:
+    """Device time of one step: HIP events on `stream` around k graph-replayed
+    steps (no host synchronisation between them)."""
+    with torch.cuda.stream(stream):
+        run(warm)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run(k)
+        e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / k
 
 
 def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1, 0.1),
@@ -426,11 +461,18 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
     """Per held-out seed: plain n^3, corrected n^3 (n = fine/4) and plain
     2n^3 runs against the 4x downsampled fine run, at the times the corrected
     run reaches after each checkpoint step count (default: nsteps); returns
-    per-seed rows with the errors at each checkpoint and GPU times."""
+    per-seed rows with the errors at each checkpoint and GPU times.
+
+    GPU time (gpu_s) is the steps a run took times its device time per step,
+    measured once (first seed) for each of the three runs with HIP events
+    around graph-replayed steps, so the three are timed alike.  wall_s is the
+    host-synchronous time of the evaluation loop itself (the plain runs
+    synchronise every step to land on the checkpoint time)."""
     rng = np.random.default_rng(seed)
     n, r = fine // 4, 4
     cps = sorted(set(int(c) for c in (checkpoints or [nsteps]) if 0 < c <= nsteps) | {nsteps})
     res = []
+    step_s = None
     fbox = solver.Box(fine, fine, fine, cfl=0.7)
     for s in seeds:
         spec = solver.ICSpec(seed=int(s), amp_v=float(rng.uniform(*amp_v)),
@@ -448,15 +490,20 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
         pc.set_state(u4, b4)
         p2 = solver.Box(2 * n, 2 * n, 2 * n, cfl=0.7)
         p2.set_state(u2, b2)
-        gpu = {"plain": 0.0, "corrected": 0.0, "plain_2x": 0.0}
+        wall = {"plain": 0.0, "corrected": 0.0, "plain_2x": 0.0}
+        steps = {"plain": 0, "corrected": 0, "plain_2x": 0}
         row = {"seed": int(s), "amp_v": spec.amp_v, "amp_a": spec.amp_a, "checkpoints": []}
         done = 0
         for c in cps:
-            gpu["corrected"] += _timed(lambda: cb.step(c - done))
+            dt_s, _ = _timed(lambda: cb.step(c - done, graph=True))
+            wall["corrected"] += dt_s
+            steps["corrected"] += c - done
             done = c
             T = cb.box.time()[0]
-            gpu["plain"] += _timed(lambda: pc.advance_to(T))
-            gpu["plain_2x"] += _timed(lambda: p2.advance_to(2 * T))
+            for key, bx, t_end in (("plain", pc, T), ("plain_2x", p2, 2 * T)):
+                dt_s, m = _timed(lambda: bx.advance_to(t_end))
+                wall[key] += dt_s
+                steps[key] += m
             fbox.advance_to(4 * T)  # the reference solution
             with torch.cuda.stream(fbox.stream):
                 uf, bf = downsample(fbox.u, fbox.b, fine, r)
@@ -466,10 +513,19 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
             row["checkpoints"].append({
                 "steps": c, "T": T, "plain": rel_error(pc.u, pc.b, uf, bf, n),
                 "corrected": rel_error(cb.box.u, cb.box.b, uf, bf, n),
-                "plain_2x": rel_error(u22, b22, uf, bf, n), "gpu_s": dict(gpu)})
-        last = row["checkpoints"][-1]
-        row.update({k: last[k] for k in ("T", "plain", "corrected", "plain_2x", "gpu_s")})
+                "plain_2x": rel_error(u22, b22, uf, bf, n), "steps_taken": dict(steps),
+                "wall_s": dict(wall)})
         row["fallbacks"] = cb.fallbacks()
+        if step_s is None:  # per-step device times, after this seed's error runs
+            step_s = {"corrected": _device_step_s(cb.box.stream, lambda m: cb.step(m, graph=True)),
+                      "plain": _device_step_s(pc.stream, lambda m: pc.run(m, graph=True)),
+                      "plain_2x": _device_step_s(p2.stream, lambda m: p2.run(m, graph=True))}
+        for q in row["checkpoints"]:
+            q["gpu_s"] = {k: q["steps_taken"][k] * step_s[k] for k in step_s}
+        row["step_ms"] = {k: 1e3 * v for k, v in step_s.items()}
+        last = row["checkpoints"][-1]
+        row.update({k: last[k] for k in ("T", "plain", "corrected", "plain_2x", "gpu_s", "wall_s",
+                                         "steps_taken")})
         row["alpha"] = cb.alpha
         row["limit"] = cb.limit
         res.append(row)
@@ -477,7 +533,8 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
             log(f"[corrector] seed {s}: " + "; ".join(
                 f"{q['steps']} steps T={q['T']:.1f} err plain {q['plain']['mean']:.3e} corrected "
                 f"{q['corrected']['mean']:.3e} 2x {q['plain_2x']['mean']:.3e}"
-                for q in row["checkpoints"]) + f"; gpu s {gpu}; {row['fallbacks']}")
+                for q in row["checkpoints"]) + f"; gpu s {row['gpu_s']}; steps {steps}; "
+                f"{row['fallbacks']}")
         cb.box.close()
         pc.close()
         p2.close()
@@ -491,6 +548,9 @@ def summarize(rows):
            "limit": rows[0]["limit"] if rows else None,
            "err_mean": {k: float(np.mean([r[k]["mean"] for r in rows])) for k in keys},
            "gpu_s_mean": {k: float(np.mean([r["gpu_s"][k] for r in rows])) for k in keys},
+           "wall_s_mean": {k: float(np.mean([r["wall_s"][k] for r in rows])) for k in keys},
+           "steps_mean": {k: float(np.mean([r["steps_taken"][k] for r in rows])) for k in keys},
+           "step_ms": rows[0]["step_ms"] if rows else None,
            "fallback_steps": int(sum(r["fallbacks"]["rejected"] for r in rows))}
     per = {}
     for r in rows:

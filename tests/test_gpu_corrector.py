@@ -241,3 +241,38 @@ def test_trained_corrector_beats_plain_coarse_step(gpu):
     r = rows[0]
     assert r["corrected"]["mean"] < 0.8 * r["plain"]["mean"]
     assert r["fallbacks"]["rejected"] <= 40
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..",
+                                                    "heterogeneous_mhd_amd", "data",
+                                                    "corrector_64.npz")),
+                    reason="no trained corrector weights in the tree")
+def test_corrected_step_graph_replays_equal_eager_steps(gpu):
+    """One captured graph per corrected step (CFL of the corrected state, the
+    step, the network, the checked correction) gives bitwise the eager
+    result on every replay, with eager work and synchronisation between the
+    replays."""
+    import torch
+    from heterogeneous_mhd_amd import corrector, solver
+    print("corrected step: graph replays vs eager, 64^3", flush=True)
+    net = corrector.CorrectorNet.load()
+    n = 64
+    u0, b0 = solver.ic_host(n, n, n, solver.ICSpec(seed=5001, amp_v=0.1, amp_a=0.05))
+    eager = corrector.CorrectedBox(net, n)
+    graph = corrector.CorrectedBox(net, n)
+    for cb in (eager, graph):
+        cb.box.set_state(torch.from_numpy(u0), torch.from_numpy(b0))
+    for rep in range(3):
+        eager.step(2, graph=False)
+        graph.step(2, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(eager.box.u, graph.box.u) and torch.equal(eager.box.b, graph.box.b), rep
+        assert eager.box.time() == graph.box.time(), rep
+    assert eager.fallbacks() == graph.fallbacks() and eager.fallbacks()["accepted"] == 6
+    # the eager form continues from a replayed state as from its own
+    eager.step(1)
+    graph.step(1)
+    torch.cuda.synchronize()
+    assert torch.equal(eager.box.u, graph.box.u) and torch.equal(eager.box.b, graph.box.b)
+    eager.box.close()
+    graph.box.close()
