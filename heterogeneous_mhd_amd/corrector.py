@@ -611,6 +611,11 @@ def main(argv=None):
     sel.add_argument("--seeds", type=int, default=2)
     sel.add_argument("--steps", type=int, default=200)
     sel.add_argument("--fine", type=int, default=256)
+    st = sub.add_parser("steptime", help="device time of a plain and of a corrected n^3 step "
+                                          "(graph replays)")
+    st.add_argument("--weights", default=str(WEIGHTS))
+    st.add_argument("--n", type=int, default=64)
+    st.add_argument("--steps", type=int, default=50)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=str(WEIGHTS))
     e.add_argument("--seeds", type=int, default=50)
@@ -709,6 +714,21 @@ def main(argv=None):
                                        "grid": grid}})
         net.save(a.out or a.weights)
         log(f"[corrector] selected alpha {al} limit {lim}; saved {a.out or a.weights}")
+    elif a.cmd == "steptime":
+        net = CorrectorNet.load(a.weights)
+        spec = solver.ICSpec(seed=5000, amp_v=0.1, amp_a=0.05)
+        cb = CorrectedBox(net, a.n)
+        cb.box.init(spec)
+        pc = solver.Box(a.n, a.n, a.n, cfl=0.7)
+        pc.init(spec)
+        out = {"n": a.n, "steps": a.steps,
+               "plain_ms": 1e3 * _device_step_s(pc.stream, lambda m: pc.run(m, graph=True), a.steps),
+               "corrected_ms": 1e3 * _device_step_s(cb.box.stream,
+                                                    lambda m: cb.step(m, graph=True), a.steps),
+               "fallbacks": cb.fallbacks()}
+        log("[corrector] steptime " + json.dumps(out))
+        cb.box.close()
+        pc.close()
     else:
         net = CorrectorNet.load(a.weights)
         rows = evaluate(net, list(range(5000, 5000 + a.seeds)), a.steps, fine=a.fine,

@@ -26,23 +26,23 @@ import sys
 
 # kernel-name prefix -> operator whose C call launches it
 _OPS = [
-    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, false, \d+, \w+, 2>"), "advect2_fluidx"),
-    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, true, \d+, \w+, 2>"), "advect2_fluidx fast"),
-    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, false[,>]"), "advect_fluidx"),
-    (re.compile(r"advect_fluid_kernel<\d+, \d+, false, true[,>]"), "advect_fluidx fast"),
-    (re.compile(r"advect_fluid_kernel<\d+, \d+, true, false[,>]"), "advect_fluidx+calcfl"),
-    (re.compile(r"advect_fluid_kernel<\d+, \d+, true, true[,>]"), "advect_fluidx+calcfl fast"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, false, false[,>]"), "fluidx"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, false, true[,>]"), "fluidx fast"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, true, false[,>]"), "fluidx+calcfl"),
-    (re.compile(r"fluidx_kernel<\d+, \d+, true, true[,>]"), "fluidx+calcfl fast"),
-    (re.compile(r"advect_kernel<\d+, \d+, false[,>]"), "advectbyzx"),
-    (re.compile(r"advect_kernel<\d+, \d+, true[,>]"), "advectbyzx fast"),
-    (re.compile(r"transpose_kernel<5>"), "transpose (u, 5 comps)"),
-    (re.compile(r"transpose_kernel<3>"), "transpose (b, 3 comps)"),
-    (re.compile(r"cfl_kernel"), "calcfl"),
-    (re.compile(r"cfl_fold_kernel"), "calcfl fold"),
-    (re.compile(r"begin_step_kernel"), "begin_step"),
+    (re.compile(r"\badvect_fluid_kernel<\d+, \d+, false, false, \d+, \w+, 2>"), "advect2_fluidx"),
+    (re.compile(r"\badvect_fluid_kernel<\d+, \d+, false, true, \d+, \w+, 2>"), "advect2_fluidx fast"),
+    (re.compile(r"\badvect_fluid_kernel<\d+, \d+, false, false[,>]"), "advect_fluidx"),
+    (re.compile(r"\badvect_fluid_kernel<\d+, \d+, false, true[,>]"), "advect_fluidx fast"),
+    (re.compile(r"\badvect_fluid_kernel<\d+, \d+, true, false[,>]"), "advect_fluidx+calcfl"),
+    (re.compile(r"\badvect_fluid_kernel<\d+, \d+, true, true[,>]"), "advect_fluidx+calcfl fast"),
+    (re.compile(r"\bfluidx_kernel<\d+, \d+, false, false[,>]"), "fluidx"),
+    (re.compile(r"\bfluidx_kernel<\d+, \d+, false, true[,>]"), "fluidx fast"),
+    (re.compile(r"\bfluidx_kernel<\d+, \d+, true, false[,>]"), "fluidx+calcfl"),
+    (re.compile(r"\bfluidx_kernel<\d+, \d+, true, true[,>]"), "fluidx+calcfl fast"),
+    (re.compile(r"\badvect_kernel<\d+, \d+, false[,>]"), "advectbyzx"),
+    (re.compile(r"\badvect_kernel<\d+, \d+, true[,>]"), "advectbyzx fast"),
+    (re.compile(r"\btranspose_kernel<5>"), "transpose (u, 5 comps)"),
+    (re.compile(r"\btranspose_kernel<3>"), "transpose (b, 3 comps)"),
+    (re.compile(r"\bcfl_kernel"), "calcfl"),
+    (re.compile(r"\bcfl_fold_kernel"), "calcfl fold"),
+    (re.compile(r"\bbegin_step_kernel"), "begin_step"),
     (re.compile(r"copyBuffer"), "copy"),
     (re.compile(r"ens_fluid_kernel<\d+, \d+, \d+, true"), "ensemble fluidx+calcfl"),
     (re.compile(r"ens_fluid_kernel"), "ensemble fluidx"),
@@ -53,6 +53,15 @@ _OPS = [
     (re.compile(r"ens_transpose_kernel<3>"), "ensemble transpose (b)"),
     (re.compile(r"ens_cfl_fold_kernel"), "ensemble calcfl fold"),
     (re.compile(r"ens_begin_kernel"), "ensemble begin_step"),
+    (re.compile(r"\bconv_kernel<0"), "corr_conv hidden (GELU)"),
+    (re.compile(r"\bconv_kernel<1"), "corr_conv output"),
+    (re.compile(r"\bconv_kernel<2"), "corr_conv input gradient"),
+    (re.compile(r"\bconv_dw_kernel"), "corr_conv_dw"),
+    (re.compile(r"\bdw_reduce_kernel"), "corr_dw_reduce"),
+    (re.compile(r"\bpack_kernel"), "corr_pack"),
+    (re.compile(r"\b(ysum|ymean|apply|check|decide|commit)_kernel"), "corr_apply"),
+    (re.compile(r"\bweight_prep_kernel"), "corr_weight_prep"),
+    (re.compile(r"\bic_pass\d"), "ic"),
 ]
 
 
