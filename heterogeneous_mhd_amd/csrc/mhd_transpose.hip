@@ -1,52 +1,94 @@
-// mhd_transpose.hip -- transpose12 / transpose13 launchers (tile code:
-// mhd_transpose_body.h).
+// mhd_transpose.hip -- transpose12 / transpose13 launchers, u and b in one
+// launch (tile code: mhd_transpose_body.h).
 #include "mhd_transpose_body.h"
 
 namespace mhd {
 namespace {
 
-// plane_fast: blockIdx.x runs over the planes and (y, z) over the tiles, so
-// the workgroups in flight share tile rows; transpose13, whose rows lie a
-// whole x-y plane apart in both arrays, uses it.
-template <int NC>
-__global__ __launch_bounds__(kTB) void transpose_kernel(float *__restrict__ out,
-                                                       const float *__restrict__ in, int ni,
-                                                       int na, size_t sa_in, size_t sz_in,
-                                                       size_t si_out, size_t sz_out, Perm perm,
-                                                       int aligned, int plane_fast)
+// One launch transposes u (5 components) and b (3 components): the plane
+// index runs over 2 x nplanes, the first half u's planes, the second b's (a
+// workgroup-uniform branch; the LDS tile is sized for u).  plane_fast:
+// blockIdx.x runs over the planes and (y, z) over the tiles, so the
+// workgroups in flight share tile rows; transpose13, whose rows lie a whole
+// x-y plane apart in both arrays, uses it.
+struct PairArgs {
+    float *out[2];
+    const float *in[2];
+    Perm perm[2];
+    int aligned[2];
+};
+
+__global__ __launch_bounds__(kTB) void transpose_kernel(PairArgs pa, int ni, int na, int nplanes,
+                                                       size_t sa_in, size_t sz_in, size_t si_out,
+                                                       size_t sz_out, int plane_fast)
 {
-    __shared__ __attribute__((aligned(16))) float tile[kT * lds_row<NC>()];
+    __shared__ __attribute__((aligned(16))) float tile[kT * lds_row<5>()];
     const int i0 = (plane_fast ? blockIdx.y : blockIdx.x) * kT;
     const int a0 = (plane_fast ? blockIdx.z : blockIdx.y) * kT;
-    const size_t zz = plane_fast ? blockIdx.x : blockIdx.z;
+    const int pz = plane_fast ? blockIdx.x : blockIdx.z;
+    const int which = pz >= nplanes;                // 0: u, 1: b
+    const size_t zz = (size_t)(pz - which * nplanes);
     const int ti = ni - i0 < kT ? ni - i0 : kT;   // cells present along i
     const int ta = na - a0 < kT ? na - a0 : kT;   // cells present along a
-    if (aligned && ti == kT && ta == kT)
-        tile_fast<NC>(out, in, tile, i0, a0, zz, sa_in, sz_in, si_out, sz_out, perm);
-    else
-        tile_scalar<NC>(out, in, tile, i0, a0, ti, ta, zz, sa_in, sz_in, si_out, sz_out, perm);
+    const bool fast = pa.aligned[which] && ti == kT && ta == kT;
+    if (which == 0) {
+        if (fast)
+            tile_fast<5>(pa.out[0], pa.in[0], tile, i0, a0, zz, sa_in, sz_in, si_out, sz_out,
+                         pa.perm[0]);
+        else
+            tile_scalar<5>(pa.out[0], pa.in[0], tile, i0, a0, ti, ta, zz, sa_in, sz_in, si_out,
+                           sz_out, pa.perm[0]);
+    } else {
+        if (fast)
+            tile_fast<3>(pa.out[1], pa.in[1], tile, i0, a0, zz, sa_in, sz_in, si_out, sz_out,
+                         pa.perm[1]);
+        else
+            tile_scalar<3>(pa.out[1], pa.in[1], tile, i0, a0, ti, ta, zz, sa_in, sz_in, si_out,
+                           sz_out, pa.perm[1]);
+    }
 }
 
 template <int NC>
-int launch_one(float *out, const float *in, int ni, int na, int nz_planes, size_t sa_in,
-               size_t sz_in, size_t si_out, size_t sz_out, const int *p, hipStream_t s,
-               bool plane_fast = false)
+Perm make_perm(const int *p)
 {
-    if (nz_planes > 65535) return MHD_ERR_ARG;
     Perm perm = {};
     for (int c = 0; c < NC; c++) {
         perm.p[c] = p[c];
         perm.inv[p[c]] = c;
     }
-    // 16-byte vectors need every row start 16-byte aligned: strides multiple of 4 cells
-    const int aligned = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0) && sa_in % 4 == 0 &&
-                        sz_in % 4 == 0 && si_out % 4 == 0 && sz_out % 4 == 0;
+    return perm;
+}
+
+// 16-byte vectors need every row start 16-byte aligned: strides multiple of 4 cells
+int aligned_rows(const float *out, const float *in, size_t sa_in, size_t sz_in, size_t si_out,
+                 size_t sz_out)
+{
+    return ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0) && sa_in % 4 == 0 &&
+           sz_in % 4 == 0 && si_out % 4 == 0 && sz_out % 4 == 0;
+}
+
+int launch_pair(float *u_out, float *b_out, const float *uu, const float *bu, int ni, int na,
+                int nplanes, size_t sa_in, size_t sz_in, size_t si_out, size_t sz_out,
+                const int *pu, const int *pb, hipStream_t s, bool plane_fast = false)
+{
     const unsigned ti = (ni + kT - 1) / kT, ta = (na + kT - 1) / kT;
-    if (plane_fast && (ti > 65535 || ta > 65535)) return MHD_ERR_ARG;
-    const dim3 grid = plane_fast ? dim3(nz_planes, ti, ta) : dim3(ti, ta, nz_planes);
+    const long np2 = 2L * nplanes;
+    if (np2 > 65535) plane_fast = true;  // only x takes more than 65535 groups
+    if (plane_fast ? (ti > 65535 || ta > 65535 || np2 > 0x7fffffffL) : ta > 65535)
+        return MHD_ERR_ARG;
+    PairArgs pa;
+    pa.out[0] = u_out;
+    pa.out[1] = b_out;
+    pa.in[0] = uu;
+    pa.in[1] = bu;
+    pa.perm[0] = make_perm<5>(pu);
+    pa.perm[1] = make_perm<3>(pb);
+    pa.aligned[0] = aligned_rows(u_out, uu, sa_in, sz_in, si_out, sz_out);
+    pa.aligned[1] = aligned_rows(b_out, bu, sa_in, sz_in, si_out, sz_out);
+    const dim3 grid = plane_fast ? dim3((unsigned)np2, ti, ta) : dim3(ti, ta, (unsigned)np2);
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
-    hipLaunchKernelGGL((transpose_kernel<NC>), grid, dim3(kTB), 0, s, out, in, ni, na, sa_in,
-                       sz_in, si_out, sz_out, perm, aligned, (int)plane_fast);
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(kTB), 0, s, pa, ni, na, nplanes, sa_in, sz_in,
+                       si_out, sz_out, (int)plane_fast);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
@@ -60,9 +102,8 @@ int launch_transpose12(float *u_out, float *b_out, const float *uu, const float 
 {
     if (!u_out || !b_out || !uu || !bu || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
     const size_t plane = (size_t)nx * ny;
-    int rc = launch_one<5>(u_out, uu, nx, ny, nz, (size_t)nx, plane, (size_t)ny, plane, kPermU12, s);
-    if (rc) return rc;
-    return launch_one<3>(b_out, bu, nx, ny, nz, (size_t)nx, plane, (size_t)ny, plane, kPermB12, s);
+    return launch_pair(u_out, b_out, uu, bu, nx, ny, nz, (size_t)nx, plane, (size_t)ny, plane,
+                       kPermU12, kPermB12, s);
 }
 
 int launch_transpose13(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
@@ -74,9 +115,8 @@ int launch_transpose13(float *u_out, float *b_out, const float *uu, const float 
     // out dims (nz, ny, nx): out cell = z + y*nz + i*nz*ny.
     const size_t sa_in = (size_t)nx * ny, sz_in = (size_t)nx;
     const size_t si_out = (size_t)nz * ny, sz_out = (size_t)nz;
-    int rc = launch_one<5>(u_out, uu, nx, nz, ny, sa_in, sz_in, si_out, sz_out, kPermU13, s, true);
-    if (rc) return rc;
-    return launch_one<3>(b_out, bu, nx, nz, ny, sa_in, sz_in, si_out, sz_out, kPermB13, s, true);
+    return launch_pair(u_out, b_out, uu, bu, nx, nz, ny, sa_in, sz_in, si_out, sz_out, kPermU13,
+                       kPermB13, s, true);
 }
 
 }  // namespace mhd

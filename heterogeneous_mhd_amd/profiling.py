@@ -38,6 +38,7 @@ _OPS = [
     (re.compile(r"\bfluidx_kernel<\d+, \d+, true, true[,>]"), "fluidx+calcfl fast"),
     (re.compile(r"\badvect_kernel<\d+, \d+, false[,>]"), "advectbyzx"),
     (re.compile(r"\badvect_kernel<\d+, \d+, true[,>]"), "advectbyzx fast"),
+    (re.compile(r"\btranspose_kernel(?!<)"), "transpose (u and b)"),
     (re.compile(r"\btranspose_kernel<5>"), "transpose (u, 5 comps)"),
     (re.compile(r"\btranspose_kernel<3>"), "transpose (b, 3 comps)"),
     (re.compile(r"\bcfl_kernel"), "calcfl"),
