@@ -171,6 +171,29 @@ def test_transposes_large_match_permutation(gpu):
     assert bits_equal(_h(tb).reshape(nx, ny, nz, 3), B.transpose(2, 1, 0, 3)[..., [2, 1, 0]])
 
 
+def test_transposes_many_planes(gpu):
+    """More than 32767 planes: the one launch for u and b then runs its
+    planes on the grid's x dimension (the only one past 65535 workgroups);
+    the result is still exactly the permutation."""
+    import torch
+    from heterogeneous_mhd_amd import ops
+    rng = np.random.default_rng(3)
+    for nx, ny, nz in ((2, 3, 40000), (3, 40000, 3)):
+        print(f"transposes, {nx}x{ny}x{nz}", flush=True)
+        u = rng.standard_normal(5 * nx * ny * nz).astype(np.float32)
+        b = rng.standard_normal(3 * nx * ny * nz).astype(np.float32)
+        ud, bd = _t(u, gpu), _t(b, gpu)
+        tu, tb = torch.empty_like(ud), torch.empty_like(bd)
+        U = u.reshape(nz, ny, nx, 5)
+        B = b.reshape(nz, ny, nx, 3)
+        ops.transpose12(tu, tb, ud, bd, nx, ny, nz)
+        assert bits_equal(_h(tu).reshape(nz, nx, ny, 5), U.transpose(0, 2, 1, 3)[..., [0, 2, 1, 3, 4]])
+        assert bits_equal(_h(tb).reshape(nz, nx, ny, 3), B.transpose(0, 2, 1, 3)[..., [1, 0, 2]])
+        ops.transpose13_any(tu, tb, ud, bd, nx, ny, nz)  # out dims (nz, ny, nx)
+        assert bits_equal(_h(tu).reshape(nx, ny, nz, 5), U.transpose(2, 1, 0, 3)[..., [0, 3, 2, 1, 4]])
+        assert bits_equal(_h(tb).reshape(nx, ny, nz, 3), B.transpose(2, 1, 0, 3)[..., [2, 1, 0]])
+
+
 def test_calcfl_nan_semantics(gpu):
     """The running max_2num of C/calcfl.c:55 restarts after a NaN cell; the
     device reduction reproduces the serial result, NaN positions included."""
