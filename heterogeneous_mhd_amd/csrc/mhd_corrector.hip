@@ -68,8 +68,8 @@ __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >=
 // Phi(z) = 0.5 erfc(-z/sqrt2), the standard normal CDF, in the form of
 // Numerical Recipes' erfcc: erfc(x) = t exp(-x^2 + P(t)), t = 1/(1 + x/2),
 // x >= 0, with our own degree-9 fit of P = log(exp(x^2) erfc(x) / t) on
-// t in [0, 1] (max |error in P| 1.0e-7, so erfc to ~1e-7 relative before
-// FP32 rounding of -x^2).  About 15 operations and one exp against ~60 for
+// t in [0, 1] (heterogeneous_mhd_amd/phi_fit.py; max |error in P| 1.0e-7,
+// so erfc to ~1e-7 relative before FP32 rounding of -x^2).  About 15 operations and one exp against ~60 for
 // erfcf; the activations it feeds are rounded to bf16.
 constexpr float kPhiP[10] = {
     -1.265512220e+00f, 1.000022661e+00f, 3.741248111e-01f,
