@@ -395,8 +395,11 @@ class CorrectedBox:
     def step(self, nsteps: int = 1, graph: bool = False):
         """nsteps corrected steps.  graph=True replays one HIP graph per step
         (the CFL of the corrected state, the step, the network and the
-        checked correction: one launch instead of ~12 at these small sizes);
-        the result is bitwise that of the eager form."""
+        checked correction: one launch instead of ~30 at these small sizes);
+        the result is bitwise that of the eager form.  The graph is captured
+        at the first graph=True call with the etol, limit and relaxation of
+        that moment; call reset_graph() after changing them (weights updated
+        in place are seen)."""
         bx = self.box
         if not graph:
             for _ in range(nsteps):
@@ -416,6 +419,12 @@ class CorrectedBox:
             for _ in range(nsteps):
                 self._graph.replay()
         bx.state_changed()
+
+    def reset_graph(self):
+        """Drop the captured corrected step (the next graph=True call
+        captures it again with the current settings)."""
+        torch.cuda.synchronize()
+        self._graph = None
 
     def fallbacks(self):
         s = self.stats.cpu().tolist()
