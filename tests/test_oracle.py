@@ -2,7 +2,8 @@
 
 * golden vectors written from the reference's own compiled C kernels
   (tests/golden/make_golden.py): bitwise, FP32 and FP64;
-* where /root/reference is present, a live re-run of the compiled reference;
+* where the reference's sources are present to build oracle/_ref, a live
+  re-run of the compiled reference;
 * closed forms and invariants chosen so that a dropped term, a wrong sign or
   index, or a transposed operand fails one of them.
 """
@@ -67,7 +68,7 @@ def test_final_state_fixture_16():
 refc_missing = not (ORACLE_DIR / "_ref" / "libmhd_refc_f32.so").exists()
 
 
-@pytest.mark.skipif(refc_missing, reason="oracle/_ref not built (no /root/reference here)")
+@pytest.mark.skipif(refc_missing, reason="oracle/_ref not built (no reference sources here)")
 @pytest.mark.parametrize("case", ["rough_12x10x10", "7x3x7"])
 def test_oracle_matches_live_reference(case):
     """Re-runs the compiled reference on fresh seeded inputs (not only the
