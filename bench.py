@@ -17,8 +17,12 @@ timed beside it ("exact").  Prints one JSON line:
   step_roofline  792 B/cell (the reference decomposition) per step time
   ensemble   the 96-box batched workload, aggregate cell-updates/s
   corrector  learned-corrector errors and GPU times on held-out seeds
+  sizes      the 64^3 and 256^3 workloads (v 0.1, A 0.05): GPU cell-updates/s
+             of the plain step, exact and fast
   cpu_baseline  the oracle restatement (bitwise equal to the reference C
              path) timed on this host's cores on a bounded 256^3 sample
+  cpu_64     the same oracle on the 64^3 workload (10 steps), serial and on
+             up to 16 threads
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--precision fast|exact]
 """
@@ -173,6 +177,49 @@ def timed_steps(box, steps: int, warmup: int, graph: bool):
     return time.perf_counter() - t0
 
 
+def size_sweep(sizes=(64, 256), steps=(50, 20)):
+    """The smaller workloads (64^3 and 256^3, v amp 0.1, A amp 0.05, CFL 0.7):
+    GPU cell-updates/s of the plain step in both precisions, from HIP events
+    on the box's stream around graph-replayed steps (state in HBM)."""
+    import torch
+    from heterogeneous_mhd_amd import solver
+    out = {}
+    for n, k in zip(sizes, steps):
+        row = {}
+        for prec in ("exact", "fast"):
+            box = solver.Box(n, n, n, cfl=0.7, precision=prec)
+            box.init(solver.ICSpec(seed=1, amp_v=0.1, amp_a=0.05))
+            box.run(3, graph=True)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(box.stream)
+            box.run(k, graph=True)
+            e1.record(box.stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / k
+            row[prec] = {"value": n ** 3 / (ms / 1e3), "ms_per_step": ms}
+            box.close()
+        out[f"{n}^3"] = dict(row, steps=k)
+    return out
+
+
+def cpu_small(n: int = 64, steps: int = 10):
+    """The oracle on the 64^3 workload (v amp 0.1, A amp 0.05, 10 steps),
+    serial and on up to 16 host threads."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from heterogeneous_mhd_amd import solver
+    from oracle import Oracle
+    u0, b0 = solver.ic_host(n, n, n, solver.ICSpec(seed=1, amp_v=0.1, amp_a=0.05))
+    out = {}
+    for threads in (1, min(16, os.cpu_count() or 1)):
+        o = Oracle("f32", threads=threads)
+        t0 = time.perf_counter()
+        o.run(u0, b0, n, n, n, steps)
+        el = time.perf_counter() - t0
+        out[f"threads_{threads}"] = {"value": n ** 3 * steps / el, "s": el}
+    out["sample"] = f"{n}^3, {steps} full steps from the IC (oracle FP32, bitwise = reference C path)"
+    return out
+
+
 def corrector_bench(seeds: int, steps: int):
     """The learned-corrector workload: held-out seeds, 4x-downsampled 256^3
     runs as the reference solution; errors of plain 64^3, corrected 64^3 and plain 128^3
@@ -204,6 +251,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-exact", action="store_true", help="skip the bitwise-exact leg")
     ap.add_argument("--no-corrector", action="store_true")
+    ap.add_argument("--no-sizes", action="store_true", help="skip the 64^3 / 256^3 throughputs")
     ap.add_argument("--corrector-seeds", type=int, default=2)
     ap.add_argument("--corrector-steps", type=int, default=100)
     ap.add_argument("--precision", choices=["exact", "fast"], default="fast")
@@ -277,6 +325,10 @@ def main():
     if not args.no_ensemble:
         print("[bench] batched ensemble", file=sys.stderr, flush=True)
         line["ensemble"] = ensemble_bench(precision=args.precision)
+    if not args.no_sizes:
+        print("[bench] 64^3 and 256^3 steps", file=sys.stderr, flush=True)
+        line["sizes"] = size_sweep()
+        torch.cuda.empty_cache()
     if not args.no_corrector:
         print("[bench] learned corrector", file=sys.stderr, flush=True)
         line["corrector"] = corrector_bench(args.corrector_seeds, args.corrector_steps)
@@ -289,6 +341,8 @@ def main():
                                                             amp_a=args.amp_a))
         torch.cuda.synchronize()
         line["cpu_baseline"] = cpu_baseline(uc.cpu().numpy(), bc.cpu().numpy(), nc)
+        print("[bench] CPU at 64^3", file=sys.stderr, flush=True)
+        line["cpu_64"] = cpu_small()
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
