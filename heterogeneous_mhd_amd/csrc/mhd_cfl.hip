@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kFoldBlock) void cfl_fold_kernel(int nrec, CflOut o
     unsigned *ticket = (unsigned *)o.ws;
     // level 1: workgroup g folds records [g*chunk, (g+1)*chunk), thread t a
     // contiguous slice of that, in order
-    const int chunk = (nrec + kFoldGroups - 1) / kFoldGroups;
+    const int chunk = (nrec + (int)gridDim.x - 1) / (int)gridDim.x;
     const int per = (chunk + kFoldBlock - 1) / kFoldBlock;
     const int lo = blockIdx.x * chunk;
     const int hi = nrec < lo + chunk ? nrec : lo + chunk;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kFoldBlock) void cfl_fold_kernel(int nrec, CflOut o
     // level 2, by the last workgroup to finish
     __threadfence();
     CflRec r2 = ident;
-    if (threadIdx.x < kFoldGroups) {
+    if (threadIdx.x < gridDim.x) {  // one level-2 record per workgroup of this launch
         r2.has_nan = __hip_atomic_load(&l2[threadIdx.x].has_nan, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         r2.after = __hip_atomic_load(&l2[threadIdx.x].after, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -117,7 +117,10 @@ static_assert(kFoldGroups == kFoldBlock, "level 2 folds one record per thread");
 int launch_cfl_fold(int nrec, const CflOut &o, hipStream_t s)
 {
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
-    hipLaunchKernelGGL(cfl_fold_kernel, dim3(kFoldGroups), dim3(kFoldBlock), 0, s, nrec, o);
+    // few records (small boxes): one workgroup, at most 16 records per
+    // thread, instead of 256 workgroups and a ticket wait for the last one
+    const unsigned groups = nrec <= 16 * kFoldBlock ? 1u : (unsigned)kFoldGroups;
+    hipLaunchKernelGGL(cfl_fold_kernel, dim3(groups), dim3(kFoldBlock), 0, s, nrec, o);
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
