@@ -375,12 +375,18 @@ class CorrectedBox:
     (with the positivity / energy-drift fallback)."""
 
     def __init__(self, net: CorrectorNet, n: int, etol: float = 1e-3, cfl: float = 0.7,
-                 alpha: float | None = None, limit: float | None = None):
+                 alpha: float | None = None, limit: float | None = None,
+                 alpha_b: float | None = None):
         self.net, self.n, self.etol = net, n, etol
         self.alpha = float(net.meta.get("alpha", 1.0) if alpha is None else alpha)
         # network outputs limited to +-limit (units of the training correction's std dev)
         self.limit = float(net.meta.get("limit", 0.0) if limit is None else limit)
+        # relaxation of the field correction (default: the same as the fluid's)
+        meta_b = net.meta.get("alpha_b")
+        self.alpha_b = float(alpha_b if alpha_b is not None
+                             else meta_b if meta_b is not None and alpha is None else self.alpha)
         self.scale = net.scale * self.alpha  # relaxation of the correction
+        self.scale[5] = net.scale[5] * self.alpha_b
         self.box = solver.Box(n, n, n, cfl=cfl)
         self.buf = Buffers(n, n, n, net.device, train=False)
         self.stats = torch.zeros(3, dtype=torch.int32, device=net.device)
@@ -466,6 +472,7 @@ def _device_step_s(stream, run, k: int = 40, warm: int = 3) -> float:
 
 def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1, 0.1),
              amp_a=(0.05, 0.05), seed: int = 99, etol: float = 1e-3, alpha: float | None = None,
+             alpha_b: float | None = None,
              checkpoints=None, limit: float | None = None, log=print):
     """Per held-out seed: plain n^3, corrected n^3 (n = fine/4) and plain
     2n^3 runs against the 4x downsampled fine run, at the times the corrected
@@ -491,7 +498,7 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
             u4, b4 = downsample(fbox.u, fbox.b, fine, 4)
             u2, b2 = downsample(fbox.u, fbox.b, fine, 2)
         fbox.stream.synchronize()
-        cb = CorrectedBox(net, n, etol=etol, alpha=alpha, limit=limit)
+        cb = CorrectedBox(net, n, etol=etol, alpha=alpha, limit=limit, alpha_b=alpha_b)
         cb.box.u.copy_(u4)
         cb.box.b.copy_(b4)
         cb.box.set_time(0.0)
@@ -536,6 +543,7 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
         row.update({k: last[k] for k in ("T", "plain", "corrected", "plain_2x", "gpu_s", "wall_s",
                                          "steps_taken")})
         row["alpha"] = cb.alpha
+        row["alpha_b"] = cb.alpha_b
         row["limit"] = cb.limit
         res.append(row)
         if log:
@@ -554,6 +562,7 @@ def evaluate(net: CorrectorNet, seeds, nsteps: int, fine: int = 256, amp_v=(0.1,
 def summarize(rows):
     keys = ("plain", "corrected", "plain_2x")
     out = {"seeds": len(rows), "alpha": rows[0]["alpha"] if rows else None,
+           "alpha_b": rows[0].get("alpha_b") if rows else None,
            "limit": rows[0]["limit"] if rows else None,
            "err_mean": {k: float(np.mean([r[k]["mean"] for r in rows])) for k in keys},
            "gpu_s_mean": {k: float(np.mean([r["gpu_s"][k] for r in rows])) for k in keys},
@@ -632,6 +641,8 @@ def main(argv=None):
     e.add_argument("--fine", type=int, default=256)
     e.add_argument("--json", default=None)
     e.add_argument("--alpha", type=float, default=None)
+    e.add_argument("--alpha-b", type=float, default=None,
+                   help="relaxation of the field correction (default: stored, else --alpha)")
     e.add_argument("--limit", type=float, default=None)
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
@@ -741,7 +752,8 @@ def main(argv=None):
     else:
         net = CorrectorNet.load(a.weights)
         rows = evaluate(net, list(range(5000, 5000 + a.seeds)), a.steps, fine=a.fine,
-                        checkpoints=(50, 100, a.steps), alpha=a.alpha, limit=a.limit, log=log)
+                        checkpoints=(50, 100, a.steps), alpha=a.alpha, limit=a.limit,
+                        alpha_b=a.alpha_b, log=log)
         summ = summarize(rows)
         log("[corrector] summary " + json.dumps(summ))
         if a.json:
