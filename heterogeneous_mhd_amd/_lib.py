@@ -112,6 +112,8 @@ SIGNATURES = {
     "mhd_corr_conv_dw": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp]),
     "mhd_corr_dw_reduce": (C.c_int, [_vp, _vp, _vp]),
     "mhd_corr_weight_prep": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mhd_corr_first_weight_elems": (C.c_int, []),
+    "mhd_corr_weight_prep_first": (C.c_int, [_vp, _vp, _vp]),
     "mhd_corr_adam": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_float, C.c_float, C.c_float,
                                 C.c_float, C.c_int, C.c_float, _vp]),
     "mhd_corr_pack": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),

@@ -103,6 +103,9 @@ class CorrectorNet:
         self.step_count = 0
         self.wf = torch.empty((LAYERS, nw), dtype=torch.int16, device=self.device)
         self.wb = torch.empty_like(self.wf)
+        # the first layer's 8-channel input in the packed form (4 taps per K-step)
+        self.wf0 = torch.empty(lib().mhd_corr_first_weight_elems(), dtype=torch.int16,
+                               device=self.device)
         # input normalisation (means, inverse std devs) and output scales
         self.norm = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0] + [1] * 8, dtype=torch.float32,
                                  device=self.device)
@@ -115,6 +118,8 @@ class CorrectorNet:
         for li in range(LAYERS):
             check(lib().mhd_corr_weight_prep(_p(self.params[li]), _p(self.wf[li]), _p(self.wb[li]),
                                              _st()), "corr_weight_prep")
+        check(lib().mhd_corr_weight_prep_first(_p(self.params[0]), _p(self.wf0), _st()),
+              "corr_weight_prep_first")
 
     def bias(self, li: int) -> torch.Tensor:
         return self.params[li, self.np - CH:]
@@ -143,8 +148,9 @@ class CorrectorNet:
         a = buf.x0
         for li in range(LAYERS):
             last = li == LAYERS - 1
-            check(L.mhd_corr_conv(_p(a), _p(self.wf[li]), _p(self.bias(li)), nx, ny, nz,
-                                  1 if last else 0, None if buf.z[li] is None else _p(buf.z[li]),
+            mode, w = (1, self.wf[li]) if last else (3, self.wf0) if li == 0 else (0, self.wf[li])
+            check(L.mhd_corr_conv(_p(a), _p(w), _p(self.bias(li)), nx, ny, nz, mode,
+                                  None if buf.z[li] is None else _p(buf.z[li]),
                                   None if last else _p(buf.a[li]),
                                   None, s), "corr_conv")
             if not last:

@@ -12,6 +12,9 @@
 // (dy+1)*3 + (dx+1) for the neighbour offset (dx,dy,dz); boundaries are
 // periodic, as the box is.
 //
+// The first layer (8 data channels) has its own packed form, see
+// conv_first_kernel.
+//
 // Forward and input-gradient kernel (implicit GEMM).  A wavefront owns 64
 // consecutive voxels of one x-row (M; two adjacent y-rows when ny is even,
 // sharing the A loads) and all 32 output channels (N).  Each
@@ -100,12 +103,46 @@ __device__ __forceinline__ float gelu_grad(float z)
     return r.phi + z * r.g * 0.39894228040143268f;  // Phi + z phi(z)
 }
 
-enum { kModeHidden = 0, kModeOutput = 1, kModeGrad = 2 };
+enum { kModeHidden = 0, kModeOutput = 1, kModeGrad = 2, kModeFirst = 3 };
 
 // the network output limited to [-lim, lim] (lim <= 0: no limit)
 __device__ __forceinline__ float ylim(float y, float lim)
 {
     return lim > 0.0f ? fminf(fmaxf(y, -lim), lim) : y;
+}
+
+// Epilogue of one 64-voxel output row whose first element is at element
+// offset vb: lane holds rows (voxels) 16*mb + 4*lg + q, column (channel)
+// 16*nb + lm, i.e. element vb + (16*mb + 4*lg + q)*32 + o.
+template <int MODE>
+__device__ __forceinline__ void conv_store(const f32x4 (&acc)[4][2], size_t vb, int lm, int lg,
+                                           const float *__restrict__ bias, float *__restrict__ zout,
+                                           u16 *__restrict__ aout, const float *__restrict__ zprev)
+{
+    float *zw = zout ? zout + vb : nullptr;
+    u16 *aw16 = aout ? aout + vb : nullptr;
+    const float *zp = zprev ? zprev + vb : nullptr;
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int o = 16 * nb + lm;
+        const float bo = MODE == kModeGrad ? 0.0f : bias[o];
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const unsigned e = (unsigned)((16 * mb + 4 * lg + q) * kC + o);
+                const float x = acc[mb][nb][q];
+                if (MODE == kModeHidden) {
+                    const float z = x + bo;
+                    if (zw) zw[e] = z;  // kept for a backward pass only
+                    aw16[e] = bf16_rne(gelu(z));
+                } else if (MODE == kModeOutput) {
+                    zw[e] = x + bo;
+                } else {
+                    aw16[e] = bf16_rne(x * gelu_grad(zp[e]));
+                }
+            }
+    }
 }
 
 constexpr int kConvWaves = 8;    // wavefronts per forward-kernel workgroup
@@ -200,36 +237,10 @@ void conv_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
         // (they would need registers beyond the 4-waves-per-SIMD budget)
         __builtin_amdgcn_sched_barrier(0);
     }
-    // lane holds rows (voxels) 16*mb + 4*lg + q, column (channel) 16*nb + lm:
-    // element offset (v - v0)*32 + o from output row r's uniform base v0*32
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const size_t vb = (((size_t)k * ny + j + r) * nx + i0) * kC;
-        float *zw = zout ? zout + vb : nullptr;
-        u16 *aw16 = aout ? aout + vb : nullptr;
-        const float *zp = zprev ? zprev + vb : nullptr;
-#pragma unroll
-        for (int nb = 0; nb < 2; nb++) {
-            const int o = 16 * nb + lm;
-            const float bo = MODE == kModeGrad ? 0.0f : bias[o];
-#pragma unroll
-            for (int mb = 0; mb < 4; mb++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const unsigned e = (unsigned)((16 * mb + 4 * lg + q) * kC + o);
-                    const float x = acc[r][mb][nb][q];
-                    if (MODE == kModeHidden) {
-                        const float z = x + bo;
-                        if (zw) zw[e] = z;  // kept for a backward pass only
-                        aw16[e] = bf16_rne(gelu(z));
-                    } else if (MODE == kModeOutput) {
-                        zw[e] = x + bo;
-                    } else {
-                        aw16[e] = bf16_rne(x * gelu_grad(zp[e]));
-                    }
-                }
-        }
-    }
+    for (int r = 0; r < R; r++)
+        conv_store<MODE>(acc[r], (((size_t)k * ny + j + r) * nx + i0) * kC, lm, lg, bias, zout, aout,
+                         zprev);
 }
 
 constexpr int kConvPF1 = 2;  // A load steps in flight, one row per wavefront
@@ -250,6 +261,72 @@ void launch_conv(int mode, const u16 *x, const u16 *w, const float *bias, int nx
     else
         hipLaunchKernelGGL((conv_kernel<kModeGrad, R, PF>), grid, block, 0, s, x, w, nullptr, nx, ny, nz,
                            nullptr, a_out, z_prev);
+}
+
+// First layer: only kIn0 = 8 of the 32 input channels carry data (the
+// physical fields; pack zero-pads the rest).  Instead of 27 K-steps of which
+// 3/4 multiply zeros, each K-step packs 4 taps x 8 channels: lane group lg
+// loads the 8 channels of tap 4g + lg, and the weights are stored in that
+// order (mhd_corr_weight_prep_first; the 28th tap slot is zero).  7 K-steps,
+// 14 KB of staged weights; hidden-layer epilogue.
+constexpr int kIn0 = 8;
+constexpr int kSteps0 = (kTaps + 3) / 4;
+constexpr int kW0Elems = kSteps0 * kC * kC;
+
+__global__ __launch_bounds__(64 * kConvWaves) __attribute__((amdgpu_waves_per_eu(4)))
+void conv_first_kernel(const u16 *__restrict__ X, const u16 *__restrict__ W,
+                       const float *__restrict__ bias, int nx, int ny, int nz,
+                       float *__restrict__ zout, u16 *__restrict__ aout)
+{
+    __shared__ __attribute__((aligned(16))) u16 wl[kW0Elems];
+    for (int f = threadIdx.x; f < kW0Elems / 8; f += 64 * kConvWaves)
+        reinterpret_cast<uint4 *>(wl)[f] = reinterpret_cast<const uint4 *>(W)[f];
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int lm = lane & 15, lg = lane >> 4;
+    const long v0 = ((long)blockIdx.x * kConvWaves + wave) * 64;  // this wavefront's 64 voxels
+    if (v0 >= (long)nx * ny * nz) return;                         // tail workgroup: no work left
+    const long row = v0 / nx;
+    const int i0 = __builtin_amdgcn_readfirstlane((int)(v0 - row * nx));
+    const int j = __builtin_amdgcn_readfirstlane((int)(row % ny));
+    const int k = __builtin_amdgcn_readfirstlane((int)(row / ny));
+
+    // A operand of K-step g for this lane: voxel block mb's 16 voxels (lm),
+    // channels 0..7 of the neighbour along tap 4g + lg (zero for the pad slot)
+    auto load_step = [&](uint4 (&av)[4], int g) {
+        const int t = 4 * g + lg;
+        const int tt = t < kTaps ? t : kTaps - 1;
+        const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
+        const u16 *rb = X + ((size_t)wrap(k + dz, nz) * ny + wrap(j + dy, ny)) * nx * kC;
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(rb + (size_t)wrap(i0 + 16 * mb + lm + dx, nx) * kC);
+            av[mb] = t < kTaps ? a : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    uint4 av[2][4];
+    load_step(av[0], 0);
+#pragma unroll
+    for (int g = 0; g < kSteps0; g++) {
+        if (g + 1 < kSteps0) load_step(av[(g + 1) & 1], g + 1);
+        bf16x8 bw[2];
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++)
+            bw[nb] = *reinterpret_cast<const bf16x8 *>(wl + (g * kC + 16 * nb + lm) * kC + 8 * lg);
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf16x8, av[g & 1][mb]), bw[nb], acc[mb][nb], 0, 0, 0);
+    }
+    conv_store<kModeHidden>(acc, (size_t)v0 * kC, lm, lg, bias, zout, aout, nullptr);
 }
 
 // Weight gradient, see the file header.  ws[g] (kParams floats per
@@ -345,6 +422,16 @@ __global__ void weight_prep_kernel(const float *__restrict__ w, u16 *__restrict_
     const u16 h = f2bf(w[e]);
     wf[e] = h;
     wb[((kTaps - 1 - t) * kC + c) * kC + o] = h;
+}
+
+// first-layer form: wp[g][o][8 q + c] = w[4 g + q][o][c] (c < 8), 0 past tap 26
+__global__ void weight_prep_first_kernel(const float *__restrict__ w, u16 *__restrict__ wp)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kW0Elems) return;
+    const int g = e / (kC * kC), o = (e / kC) % kC, kk = e % kC;
+    const int t = 4 * g + kk / kIn0, c = kk % kIn0;
+    wp[e] = t < kTaps ? f2bf(w[(t * kC + o) * kC + c]) : (u16)0;
 }
 
 __global__ void adam_kernel(float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
@@ -679,12 +766,31 @@ int mhd_corr_conv(const uint16_t *x, const uint16_t *w, const float *bias, int n
     case kModeHidden: if (!bias || !a_out) return MHD_ERR_ARG; break;
     case kModeOutput: if (!bias || !z_out) return MHD_ERR_ARG; break;
     case kModeGrad: if (!a_out || !z_prev) return MHD_ERR_ARG; break;
+    case kModeFirst: if (!bias || !a_out) return MHD_ERR_ARG; break;
     default: return MHD_ERR_ARG;
     }
     (void)hipGetLastError();
     const hipStream_t s = (hipStream_t)stream;
-    if (ny % 2 == 0) launch_conv<2, kConvPF2>(mode, x, w, bias, nx, ny, nz, z_out, a_out, z_prev, s);
-    else launch_conv<1, kConvPF1>(mode, x, w, bias, nx, ny, nz, z_out, a_out, z_prev, s);
+    if (mode == kModeFirst) {
+        const long waves = (long)nx * ny * nz / 64;
+        hipLaunchKernelGGL(conv_first_kernel, dim3((unsigned)((waves + kConvWaves - 1) / kConvWaves)),
+                           dim3(64 * kConvWaves), 0, s, x, w, bias, nx, ny, nz, z_out, a_out);
+    } else if (ny % 2 == 0) {
+        launch_conv<2, kConvPF2>(mode, x, w, bias, nx, ny, nz, z_out, a_out, z_prev, s);
+    } else {
+        launch_conv<1, kConvPF1>(mode, x, w, bias, nx, ny, nz, z_out, a_out, z_prev, s);
+    }
+    return done();
+}
+
+int mhd_corr_first_weight_elems(void) { return kW0Elems; }
+
+int mhd_corr_weight_prep_first(const float *w, uint16_t *wp, mhd_stream_t stream)
+{
+    if (!w || !wp) return MHD_ERR_ARG;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(weight_prep_first_kernel, dim3(blocks(kW0Elems, 256)), dim3(256), 0,
+                       (hipStream_t)stream, w, wp);
     return done();
 }
 

@@ -275,7 +275,11 @@ int mhd_corr_num_params(void);
  *         z_out may be NULL when no backward pass follows (not stored);
  * mode 1: z_out = conv(x) + bias;
  * mode 2: a_out = bf16(conv(x) * GELU'(z_prev)) -- with the input-gradient
- *         weights of mhd_corr_weight_prep, the backward pass of a layer. */
+ *         weights of mhd_corr_weight_prep, the backward pass of a layer;
+ * mode 3: as mode 0 for the first layer, whose input has 8 data channels
+ *         (mhd_corr_pack) -- with the packed weights of
+ *         mhd_corr_weight_prep_first: 4 taps per 32-wide MFMA K-step, 7
+ *         K-steps instead of 27. */
 int mhd_corr_conv(const uint16_t *x, const uint16_t *w, const float *bias, int nx, int ny, int nz,
                   int mode, float *z_out, uint16_t *a_out, const float *z_prev, mhd_stream_t stream);
 /* Weight gradient of one layer: partial sums of dW[t][o][c] = sum_v dy[v][o] *
@@ -289,6 +293,11 @@ int mhd_corr_conv_dw(const uint16_t *dy, const uint16_t *x, int nx, int ny, int 
 int mhd_corr_dw_reduce(const float *ws, float *grad, mhd_stream_t stream);
 /* FP32 master weights -> bf16 forward form wf and input-gradient form wb. */
 int mhd_corr_weight_prep(const float *w, uint16_t *wf, uint16_t *wb, mhd_stream_t stream);
+/* FP32 master weights of the first layer -> the packed bf16 form of mode 3
+ * (mhd_corr_first_weight_elems() elements: [7][32 out][4 taps x 8 channels],
+ * the slot past tap 26 zero). */
+int mhd_corr_first_weight_elems(void);
+int mhd_corr_weight_prep_first(const float *w, uint16_t *wp, mhd_stream_t stream);
 /* Adam step on n parameters (gradient scaled by gscale, then zeroed). */
 int mhd_corr_adam(float *p, float *g, float *m, float *v, int n, float lr, float beta1, float beta2,
                   float eps, int step, float gscale, mhd_stream_t stream);

@@ -94,6 +94,62 @@ def test_conv_forward_and_input_gradient(gpu, shape, wscale):
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_first_layer_packed_conv(gpu, shape):
+    """Mode 3: the first layer with 8 data channels, 4 taps per K-step.  The
+    packed weights follow [g][o][8q + c] = w[4g + q][o][c] (zero past tap 26);
+    the result matches the oracle convolution of the zero-padded input and the
+    standard path (same sums, other order: 1e-5), with and without the
+    pre-activation store."""
+    import torch
+    from heterogeneous_mhd_amd._lib import check, lib
+    nx, ny, nz = shape
+    print(f"first-layer conv {shape}", flush=True)
+    v = nx * ny * nz
+    x = np.zeros((nz, ny, nx, 32))
+    x[..., :8] = R.bf16(RNG.standard_normal((nz, ny, nx, 8)))
+    w = RNG.standard_normal((27, 32, 32)) * 0.15  # channels 8..31 meet zero inputs
+    bias = RNG.standard_normal(32) * 0.1
+    params = np.concatenate([w.reshape(-1), bias]).astype(np.float32)
+    p = _dev(params)
+    L = lib()
+    s = torch.cuda.current_stream().cuda_stream
+    n0 = L.mhd_corr_first_weight_elems()
+    assert n0 == 7 * 32 * 32
+    wp = torch.empty(n0, dtype=torch.int16, device="cuda")
+    check(L.mhd_corr_weight_prep_first(p.data_ptr(), wp.data_ptr(), s), "prep first")
+    wq = R.bf16(params[:27 * 1024]).reshape(27, 32, 32)
+    want = np.zeros((7, 32, 4, 8))
+    for g in range(7):
+        for q in range(4):
+            if 4 * g + q < 27:
+                want[g, :, q, :] = wq[4 * g + q, :, :8]
+    assert np.array_equal(_from_bits(wp.cpu().numpy()).reshape(7, 32, 4, 8), want)
+    xd = _dev(_bits(x).reshape(-1))
+    z = torch.empty(v * 32, dtype=torch.float32, device="cuda")
+    a = torch.empty(v * 32, dtype=torch.int16, device="cuda")
+    check(L.mhd_corr_conv(xd.data_ptr(), wp.data_ptr(), p[27 * 1024:].data_ptr(), nx, ny, nz, 3,
+                          z.data_ptr(), a.data_ptr(), None, s), "conv first")
+    zr = R.conv(x, wq, params[27 * 1024:].astype(np.float64))
+    zg = _grid(z.cpu().numpy(), nx, ny, nz, 32)
+    assert np.abs(zg - zr).max() <= 1e-5 * np.abs(zr).max()
+    ag = _from_bits(_grid(a.cpu().numpy(), nx, ny, nz, 32))
+    ar = R.bf16(R.gelu(zg))
+    assert np.all(np.abs(ag - ar) <= 2.0 ** -7 * np.abs(ar) + 1e-7)
+    a2 = torch.empty_like(a)
+    check(L.mhd_corr_conv(xd.data_ptr(), wp.data_ptr(), p[27 * 1024:].data_ptr(), nx, ny, nz, 3,
+                          None, a2.data_ptr(), None, s), "conv first no z")
+    assert torch.equal(a2, a)
+    # the standard path over all 32 channels
+    wf = torch.empty(27 * 1024, dtype=torch.int16, device="cuda")
+    wb = torch.empty_like(wf)
+    check(L.mhd_corr_weight_prep(p.data_ptr(), wf.data_ptr(), wb.data_ptr(), s), "prep")
+    z0 = torch.empty_like(z)
+    check(L.mhd_corr_conv(xd.data_ptr(), wf.data_ptr(), p[27 * 1024:].data_ptr(), nx, ny, nz, 0,
+                          z0.data_ptr(), a2.data_ptr(), None, s), "conv")
+    assert np.abs(_grid(z0.cpu().numpy(), nx, ny, nz, 32) - zg).max() <= 1e-5 * np.abs(zr).max()
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 def test_weight_gradient_and_accumulation(gpu, shape):
     import torch
     from heterogeneous_mhd_amd._lib import check, lib
@@ -145,7 +201,13 @@ def test_network_gradients_match_oracle(gpu):
                 norm[:8].astype(np.float64), norm[8:].astype(np.float64))
     yr, cache = onet.forward(x0)
     yg = y.cpu().numpy().reshape(nz, ny, nx, 32)
-    assert np.abs(yg - yr).max() <= 1e-3 * np.abs(yr).max()
+    # The GPU sums in FP32 in another order than the FP64 oracle, so a few
+    # pre-activations land on the other side of a bf16 rounding boundary.
+    # Those one-ulp flips carry through the next layers: relative noise of
+    # 1e-7 on the oracle's own pre-activations moves its output by up to
+    # 2.1e-3 of max|y| (median 5.6e-4, 40 random networks and states of this
+    # shape).  A wrong weight, layer or packing moves it by O(1).
+    assert np.abs(yg - yr).max() <= 4e-3 * np.abs(yr).max()
     lr_, g = R.loss_and_grad(yr, tgt.astype(np.float32).astype(np.float64))
     assert abs(loss - lr_) <= 1e-4 * abs(lr_)
     og = onet.backward(g, cache)
