@@ -54,6 +54,7 @@ _OPS = [
     (re.compile(r"ens_transpose_kernel<3>"), "ensemble transpose (b)"),
     (re.compile(r"ens_cfl_fold_kernel"), "ensemble calcfl fold"),
     (re.compile(r"ens_begin_kernel"), "ensemble begin_step"),
+    (re.compile(r"\bconv_first_kernel"), "corr_conv first layer (GELU)"),
     (re.compile(r"\bconv_kernel<0"), "corr_conv hidden (GELU)"),
     (re.compile(r"\bconv_kernel<1"), "corr_conv output"),
     (re.compile(r"\bconv_kernel<2"), "corr_conv input gradient"),

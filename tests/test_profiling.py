@@ -28,6 +28,7 @@ def test_corrector_kernels():
     assert op_of(ANON + "conv_kernel<1, 2, 1>(unsigned short const*)") == "corr_conv output"
     assert op_of(ANON + "conv_kernel<2, 1, 2>(unsigned short const*)") == "corr_conv input gradient"
     assert op_of(ANON + "conv_dw_kernel(unsigned short const*)") == "corr_conv_dw"
+    assert op_of(ANON + "conv_first_kernel(unsigned short const*)") == "corr_conv first layer (GELU)"
     for k in ("ysum", "ymean", "apply", "check", "decide", "commit"):
         assert op_of(ANON + f"{k}_kernel(float const*)") == "corr_apply"
     assert op_of("at::native::vectorized_elementwise_kernel<4>(int)") is None
