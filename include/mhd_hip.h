@@ -276,9 +276,10 @@ int mhd_corr_num_params(void);
  * mode 1: z_out = conv(x) + bias;
  * mode 2: a_out = bf16(conv(x) * GELU'(z_prev)) -- with the input-gradient
  *         weights of mhd_corr_weight_prep, the backward pass of a layer;
- * mode 3: as mode 0 for the first layer, whose input has 8 data channels
- *         (mhd_corr_pack) -- with the packed weights of
- *         mhd_corr_weight_prep_first: 4 taps per 32-wide MFMA K-step, 7
+ * mode 3: as mode 0 (z_out may be NULL) for the first layer, whose input
+ *         carries data in channels 0..7 only (mhd_corr_pack): reads those 8
+ *         channels of x, with the packed weights of
+ *         mhd_corr_weight_prep_first -- 4 taps per 32-wide MFMA K-step, 7
  *         K-steps instead of 27. */
 int mhd_corr_conv(const uint16_t *x, const uint16_t *w, const float *bias, int nx, int ny, int nz,
                   int mode, float *z_out, uint16_t *a_out, const float *z_prev, mhd_stream_t stream);
