@@ -54,8 +54,12 @@ def _report(name, n, steps, spec, threads=16):
     from heterogeneous_mhd_amd import solver
     from oracle import Oracle
     u0, b0 = solver.ic_host(n, n, n, spec)
+
+    def progress(s, u, b, dt):
+        if (s + 1) % 5 == 0:
+            print(f"  FP64 oracle {name}: step {s + 1}/{steps}", flush=True)
     r64 = Oracle("f64", threads=threads).run(u0.astype(np.float64), b0.astype(np.float64), n, n, n,
-                                            steps)
+                                            steps, on_step=progress)
     exact, divb_e = _run_gpu(u0, b0, n, steps, "exact")
     fast, divb_f = _run_gpu(u0, b0, n, steps, "fast")
     rep = {"workload": name, "n": n, "steps": steps, "amp_v": spec.amp_v, "amp_a": spec.amp_a,
