@@ -21,8 +21,8 @@ timed beside it ("exact").  Prints one JSON line:
              of the plain step, exact and fast
   cpu_baseline  the oracle restatement (bitwise equal to the reference C
              path) timed on this host's cores on a bounded 256^3 sample
-  cpu_64     the same oracle on the 64^3 workload (10 steps), serial and on
-             up to 16 threads
+             (.small_64: the same oracle on the 64^3 workload, 10 steps,
+             serial and on up to 16 threads)
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--precision fast|exact]
 """
@@ -342,7 +342,7 @@ def main():
         torch.cuda.synchronize()
         line["cpu_baseline"] = cpu_baseline(uc.cpu().numpy(), bc.cpu().numpy(), nc)
         print("[bench] CPU at 64^3", file=sys.stderr, flush=True)
-        line["cpu_64"] = cpu_small()
+        line["cpu_baseline"]["small_64"] = cpu_small()
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
