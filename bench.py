@@ -11,9 +11,14 @@ against the reference by tests/test_gpu_fast.py); the bitwise-exact mode is
 timed beside it ("exact").  Prints one JSON line:
 
   value      cell-updates/s = N * steps / wall time of the timed steps
-  roofline   the dominant kernel's algorithmic bytes per launch / its mean
-             duration (HIP events on the launch stream) vs 8 TB/s; traffic
-             from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+  roofline   the dominant kernel (largest time per step, by rocprofv3 kernel
+             identity: both transposes are one transpose_kernel): its
+             algorithmic bytes per launch / its mean launch duration (HIP
+             events on the launch stream) vs 8 TB/s; traffic from the
+             committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+  kernels    the same figures for every kernel of the step
+  copy_GBs   the library's streaming copy (copy_matrix) on the state: the
+             achievable ceiling next to the vendor peak
   step_roofline  792 B/cell (the reference decomposition) per step time
   ensemble   the 96-box batched workload, aggregate cell-updates/s
   corrector  learned-corrector errors and GPU times on held-out seeds
@@ -47,6 +52,19 @@ CALLS_PER_STEP = {"fluidx": 3, "advectbyzx": 2, "transpose12": 2, "transpose13":
                   "advect_fluidx": 2, "advect2_fluidx": 1}
 # the reference decomposition the step roofline is priced on (792 B/cell)
 REF_CALLS = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13": 2, "calcfl": 1}
+# the kernel (as rocprofv3 names it) each operator launches, and that
+# kernel's operator key in the committed traffic summary; one kernel can serve
+# several operators (both transposes are one transpose_kernel)
+KERNEL_OF = {"fluidx": "fluidx_kernel", "advectbyzx": "advect_kernel",
+             "advect_fluidx": "advect_fluid_kernel (1 pass)",
+             "advect2_fluidx": "advect_fluid_kernel (2 passes)",
+             "transpose12": "transpose_kernel", "transpose13": "transpose_kernel",
+             "calcfl": "cfl_kernel"}
+TRAFFIC_KEY = {"fluidx_kernel": "fluidx", "advect_kernel": "advectbyzx",
+               "advect_fluid_kernel (1 pass)": "advect_fluidx",
+               "advect_fluid_kernel (2 passes)": "advect2_fluidx",
+               "transpose_kernel": "transpose (u and b)", "cfl_kernel": "calcfl"}
+PRECISION_INDEPENDENT = {"transpose_kernel", "cfl_kernel"}
 METRIC = "cell-updates/s & % HBM roofline per 3D step"
 
 
@@ -121,9 +139,34 @@ def pmc_traffic(op: str, cells: int):
     return None
 
 
+def kernel_table(kt: dict, cells: int, precision: str):
+    """Per kernel (rocprofv3 identity): launches per step, time per step, the
+    mean launch, algorithmic bytes per launch and the achieved fraction of
+    the HBM peak.  Operators that launch the same kernel are pooled."""
+    groups = {}
+    for op, t in kt.items():
+        c = CALLS_PER_STEP.get(op, 0)
+        if not c:
+            continue
+        g = groups.setdefault(KERNEL_OF[op], {"ops": [], "launches": 0, "s": 0.0, "bytes": 0.0})
+        g["ops"].append(op)
+        g["launches"] += c
+        g["s"] += c * t
+        g["bytes"] += c * ALGO_BYTES[op] * cells
+    out = {}
+    for k, g in groups.items():
+        gbs = g["bytes"] / g["s"] / 1e9
+        key = TRAFFIC_KEY[k] + ("" if precision == "exact" or k in PRECISION_INDEPENDENT else " fast")
+        out[k] = {"ops": g["ops"], "launches_per_step": g["launches"], "ms_per_step": 1e3 * g["s"],
+                  "mean_launch_s": g["s"] / g["launches"], "bytes_per_launch": g["bytes"] / g["launches"],
+                  "achieved_GBs": gbs, "frac": gbs / PEAK_HBM_GBS, "traffic": pmc_traffic(key, cells)}
+    return out
+
+
 def copy_probe(box, reps: int = 3):
-    """Achieved bandwidth of a plain device copy of the state (copy_matrix):
-    64 bytes per cell moved.  Reported next to the vendor peak."""
+    """Achieved bandwidth of the library's streaming copy (copy_matrix: u and
+    b in one launch, 16-byte loads, nontemporal stores) on the state: 64
+    bytes per cell moved.  Reported next to the vendor peak."""
     import torch
     from heterogeneous_mhd_amd import ops
     n = box.nx
@@ -283,9 +326,9 @@ def main():
     print(f"[bench] {ms_per_step:.2f} ms/step, t={t_sim:.4f} dt={dt:.4g}", file=sys.stderr, flush=True)
 
     kt = kernel_times(box, args.kernel_reps, args.precision)
-    per_step = {k: kt[k] * CALLS_PER_STEP.get(k, 0) for k in kt}
-    dom = max(per_step, key=per_step.get)
-    achieved = ALGO_BYTES[dom] * N / kt[dom] / 1e9
+    kernels = kernel_table(kt, N, args.precision)
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    dk = kernels[dom]
     step_bytes = sum(ALGO_BYTES[k] * REF_CALLS[k] for k in REF_CALLS) * N
 
     def step_roofline(w):
@@ -303,14 +346,15 @@ def main():
         "config": {"workload": f"plain relaxing-TVD split MHD step, {n}^3 periodic FP32 box, "
                                f"v amp {args.amp_v}, A amp {args.amp_a}, CFL {args.cfl}",
                    "nx": n, "ny": n, "nz": n, "graph": graph, "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": pmc_traffic(dom + (" fast" if args.precision == "fast" else ""), N),
-                     "algo_bytes_per_cell": ALGO_BYTES[dom], "mean_launch_s": kt[dom]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBs"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": dk["frac"], "traffic": dk["traffic"],
+                     "algo_bytes_per_launch": dk["bytes_per_launch"], "mean_launch_s": dk["mean_launch_s"],
+                     "launches_per_step": dk["launches_per_step"], "ops": dk["ops"]},
         "step_roofline": step_roofline(wall),
+        "kernels": kernels,
         "kernel_ms": {k: 1e3 * v for k, v in kt.items()},
         "kernel_GBs": {k: ALGO_BYTES[k] * N / v / 1e9 for k, v in kt.items()},
-        "copy_GBs": copy_gbs,
+        "copy_GBs": copy_gbs, "copy_frac": copy_gbs / PEAK_HBM_GBS,
         "divb_max": diag["divb_max"],
     }
     if args.precision == "fast" and not args.no_exact:

@@ -20,7 +20,7 @@ LIB = PKG / "libmhd_hip.so"
 
 SOURCES = ["mhd_fluid.hip", "mhd_advect.hip", "mhd_transpose.hip", "mhd_cfl.hip", "mhd_ic.hip",
            "mhd_capi.hip", "mhd_ensemble.hip", "mhd_corrector.hip",
-           "mhd_fused.hip", "mhd_io.hip"]
+           "mhd_fused.hip", "mhd_io.hip", "mhd_copy.hip"]
 
 # -ffp-contract=off: no FMA contraction, so float/double rounding follows
 # the reference's C promotion rules exactly (bitwise parity).
@@ -40,10 +40,35 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def build_hip(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / s) for s in SOURCES]
+def build_hip(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compiles each source to an object in parallel (separate compilation:
+    every kernel is launched from the file that defines it), then links."""
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    hc = hipcc()
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    cmds = []
+    for s in SOURCES:
+        obj = objdir / (Path(s).stem + ".o")
+        cmds.append([hc, *compile_flags, "-I", str(ROOT / "include"), "-c", str(CSRC / s), "-o", str(obj)])
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1, 16)
+    procs: list[tuple[list[str], subprocess.Popen]] = []
+    failed = []
+    pending = list(cmds)
+    while pending or procs:
+        while pending and len(procs) < jobs:
+            cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd)))
+        cmd, p = procs.pop(0)
+        if p.wait() != 0:
+            failed.append(cmd[-3])
+    if failed:
+        raise RuntimeError(f"hipcc failed on: {', '.join(failed)}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), *srcs, "-o", str(tmp)]
+    objs = [str(objdir / (Path(s).stem + ".o")) for s in SOURCES]
+    cmd = [hc, *HIPCC_FLAGS, *objs, "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -149,6 +149,11 @@ void copy_matrix(float *u, float *b, int *nx, int *ny, int *nz, float *update_u,
 {
     if (!dims_ok(nx, ny, nz) || !u || !b || !update_u || !update_b) return;
     const size_t n = (size_t)*nx * *ny * *nz;
+    if (update_u != u && update_b != b && is_device_ptr(u) && is_device_ptr(b) && is_device_ptr(update_u) &&
+        is_device_ptr(update_b)) {
+        finish(mhd::stream_copy2(update_u, u, 5 * n, update_b, b, 3 * n, 0), "copy_matrix");
+        return;
+    }
     if (update_u != u) note(hipMemcpy(update_u, u, 5 * n * sizeof(float), hipMemcpyDefault), "copy_matrix u");
     if (update_b != b) note(hipMemcpy(update_b, b, 3 * n * sizeof(float), hipMemcpyDefault), "copy_matrix b");
 }
@@ -326,10 +331,9 @@ int mhd_copy_matrix_async(const float *u, const float *b, int nx, int ny, int nz
 {
     if (!u || !b || !update_u || !update_b || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
     const size_t n = (size_t)nx * ny * nz;
-    hipError_t e = hipMemcpyAsync(update_u, u, 5 * n * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(update_b, b, 3 * n * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream);
-    note(e, "copy_matrix_async");
-    return e == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+    const int rc = mhd::stream_copy2(update_u, u, 5 * n, update_b, b, 3 * n, (hipStream_t)stream);
+    if (rc != MHD_OK) note(hipGetLastError(), "copy_matrix_async");
+    return rc;
 }
 
 size_t mhd_calcfl_workspace_bytes(int nx, int ny, int nz)
@@ -448,6 +452,8 @@ int mhd_plan_set_precision(mhd_plan *p, int mode)
         p->exec = nullptr;
         p->graph = nullptr;
     }
+    if (p->fast != (mode == MHD_PREC_FAST))
+        p->ready_for[0] = p->ready_for[1] = nullptr;  // the next run starts from calcfl of the state
     p->fast = mode == MHD_PREC_FAST;
     return MHD_OK;
 }

@@ -66,20 +66,52 @@ __device__ __forceinline__ float cfl_cell(const float *q, const float *bb)
     return (float)((double)v + sqrt(fabs(((double)bsq * 2.0 + (double)(gam * p)) / (double)rho)));
 }
 
-// cfl_cell in FP32 only (the tolerance mode's fused CFL): hardware
-// reciprocal and square root, FMA; max_i |m_i/rho| as max_i |m_i| * |1/rho|.
-// NaN inputs still give NaN (through msq, rho or the energy term).
-__device__ __forceinline__ float cfl_cell_fast(const float *q, const float *bb)
+// Screen for the exact maximum (the tolerance mode's fused CFL): an FP32
+// estimate f of cfl_cell with hardware reciprocal and square root and FMA,
+// or +inf for a cell outside the screened domain D below.
+//
+// Claim: for a cell in D, |f - e| <= 2^-12 e, with e = cfl_cell(q, bb).
+// Both f and e approximate the real-valued V + sqrt|X|, with
+//   V = max|m_i|/rho, K = msq/rho, P = (E - K/2)(g-1) + (1-g/2)B^2 - B^2/2,
+//   X = (2B^2 + g P)/rho, g = (float)(5/3) (g-1 and 1-g/2 are exact floats).
+// With u = 2^-24 and W = (2/3)(|E| + K/2 + B^2) >= |P|, S = (2B^2 + g W)/rho
+// (every rounding error of X is a few u times a term bounded by S):
+//   fast path (v_rcp_f32 within 2u, FMA chain): |Xf - X| <= 24 u S;
+//   exact path (float divisions, double sums, float stores of ps, p and
+//   g*p): |Xe - X| <= 12 u S.
+// D asks x = |Xf| >= S/64 (no severe cancellation in X), so
+//   |sqrt|Xf| - sqrt|Xe|| <= 36 u S / sqrt|Xf| <= 36 * 64 u sqrt|Xf| (+ 2u for
+//   v_sqrt_f32), and V is within 3.1u in both paths: |f - e| <= ~2310 u e < 2^-12 e.
+// D also asks 2^-30 <= rho <= 2^30, every other input finite with sum of
+// magnitudes <= 2^30 (no overflow, accurate reciprocal; NaN and inf fail
+// these tests) and S >= 2^-90 (underflow errors negligible); then e is
+// finite, never NaN.
+// A workgroup takes T = F (1 - 2^-8) with F the largest f of its cells: every
+// cell whose e is the workgroup's maximum has f >= E (1 - 2^-12) > T, so the
+// exact cfl_cell evaluated on the cells with f >= T alone gives the exact
+// maximum; a workgroup with any cell outside D (F = +inf) evaluates all its
+// cells exactly (fluid_group).  tests/test_cfl_screen.py checks the bound on
+// the CPU.
+__device__ __forceinline__ float cfl_screen(const float *q, const float *bb)
 {
     const float gam = 5.0f / 3.0f;
-    const float rr = __builtin_amdgcn_rcpf(q[0]);
-    const float v = fmaxf(fmaxf(fabsf(q[1]), fabsf(q[2])), fabsf(q[3])) * fabsf(rr);
+    const float rho = q[0];
+    const float rr = __builtin_amdgcn_rcpf(rho);
+    const float a1 = fabsf(q[1]), a2 = fabsf(q[2]), a3 = fabsf(q[3]), a4 = fabsf(q[4]);
+    const float v = fmaxf(fmaxf(a1, a2), a3) * rr;
     const float msq = fmaf(q[1], q[1], fmaf(q[2], q[2], q[3] * q[3]));
     const float bsq = fmaf(bb[0], bb[0], fmaf(bb[1], bb[1], bb[2] * bb[2]));
     const float ps = fmaf(fmaf(-0.5f * msq, rr, q[4]), gam - 1.0f, (1.0f - 0.5f * gam) * bsq);
     const float p = fmaf(-0.5f, bsq, ps);
-    return v + __builtin_amdgcn_sqrtf(fabsf(fmaf(2.0f, bsq, gam * p) * rr));
+    const float x = fabsf(fmaf(2.0f, bsq, gam * p) * rr);
+    const float w = (2.0f / 3.0f) * (fmaf(0.5f * msq, rr, a4) + bsq);
+    const float S = fmaf(gam, w, 2.0f * bsq) * rr;
+    const float mag = ((a1 + a2) + (a3 + a4)) + ((fabsf(bb[0]) + fabsf(bb[1])) + fabsf(bb[2]));
+    const bool in = rho >= 0x1p-30f && rho <= 0x1p30f && mag <= 0x1p30f && S >= 0x1p-90f &&
+                    x >= 0x1p-6f * S;
+    return in ? v + __builtin_amdgcn_sqrtf(x) : INFINITY;
 }
+constexpr float kCflScreenKeep = 1.0f - 0x1p-8f;  // T = F * kCflScreenKeep
 
 template <int BLOCK>
 __device__ __forceinline__ int cfl_block_max_int(int x, int *sh)

@@ -604,7 +604,10 @@ int mhd_ensemble_set_config(mhd_ensemble *e, int shape, int grid)
 int mhd_ensemble_set_precision(mhd_ensemble *e, int mode)
 {
     if (!e || (mode != MHD_PREC_EXACT && mode != MHD_PREC_FAST)) return MHD_ERR_ARG;
-    if (e->fast != (mode == MHD_PREC_FAST)) drop_graph(e);
+    if (e->fast != (mode == MHD_PREC_FAST)) {
+        drop_graph(e);
+        e->dt_ready = false;  // the next run starts from calcfl of the state, as the reference does
+    }
     e->fast = mode == MHD_PREC_FAST;
     return MHD_OK;
 }

@@ -42,8 +42,9 @@
 //
 // With a CFL workspace the kernel also computes calcfl (C/calcfl.c:13-63) of
 // the state it writes, so the last sweep of a step yields the next step's dt
-// without another pass over HBM (in FAST, in FP32 with the hardware
-// reciprocal and square root: dt within a few ulp of the exact one).
+// without another pass over HBM.  The result is bitwise calcfl's in both
+// modes; FAST screens the cells with an FP32 estimate of proven accuracy and
+// evaluates the exact bound only where it can be the maximum (cfl_screen).
 #pragma once
 #include "mhd_cfl_common.h"
 
@@ -501,12 +502,42 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
                 Bp[5 * i + c] = un[c];
             }
             if (CFL) {
-                tcf[r] = FAST ? cfl_cell_fast(un, bf[r]) : cfl_cell(un, bf[r]);
-                if (tcf[r] != tcf[r]) lnan = toff[r] > lnan ? toff[r] : lnan;
+                if constexpr (FAST) {
+                    tcf[r] = cfl_screen(un, bf[r]);  // >= 0 or +inf, see below
+                } else {
+                    tcf[r] = cfl_cell(un, bf[r]);
+                    if (tcf[r] != tcf[r]) lnan = toff[r] > lnan ? toff[r] : lnan;
+                }
             }
         }
     }
     __syncthreads();
+    if constexpr (CFL && FAST) {
+        // The exact maximum from the screened estimates (cfl_screen): the
+        // exact cfl_cell runs only on cells within 2^-8 of the workgroup's
+        // largest estimate, or on every cell when one lies outside the
+        // screened domain (F = +inf; NaN cells are found there).  The new u
+        // is read back from the staged output in B.
+        float fm = 0.0f;
+#pragma unroll
+        for (int r = 0; r < M; r++) fm = tcf[r] > fm ? tcf[r] : fm;  // -inf for absent cells
+        __shared__ float shs[BLOCK / 64];
+        const float F = cfl_block_max_f<BLOCK>(fm, shs);
+        const float T = F == INFINITY ? -INFINITY : F * kCflScreenKeep;
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const int i = lt + r * tp;
+            if (live && i < nx && tcf[r] >= T) {
+                float un[5];
+#pragma unroll
+                for (int c = 0; c < 5; c++) un[c] = Bp[5 * i + c];
+                tcf[r] = cfl_cell(un, bf[r]);
+                if (tcf[r] != tcf[r]) lnan = toff[r] > lnan ? toff[r] : lnan;
+            } else {
+                tcf[r] = -INFINITY;
+            }
+        }
+    }
     {
         float *go = u_out + (size_t)pen0 * nx * 5;
         if (vec) {

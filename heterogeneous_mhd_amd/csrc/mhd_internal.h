@@ -133,6 +133,11 @@ size_t fluid_workgroups(int nx, long npen);
 int launch_calcfl(float *inv_c, const float *u, const float *b, int nx, int ny, int nz, void *ws,
                   double cfl, float *dt_dev, float *t_dev, float tf, hipStream_t s);
 
+// copy_matrix on the device (mhd_copy.hip): dst0 <- src0 (n0 floats) and
+// dst1 <- src1 (n1 floats) in one streaming launch.
+int stream_copy2(float *dst0, const float *src0, size_t n0, float *dst1, const float *src1, size_t n1,
+                 hipStream_t s);
+
 // Seeded initial condition (mhd_ic.hip).
 int ic_mode_count(int kmax);
 int ic_generate_host(float *u, float *b, int nx, int ny, int nz, uint64_t seed, double amp_v,

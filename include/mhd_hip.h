@@ -80,7 +80,11 @@ typedef void *mhd_stream_t; /* a hipStream_t; NULL is the null stream */
  * computes in FP32 only (hardware reciprocal and square root, FMA); its
  * fields differ from the reference's by FP32 rounding noise, within the
  * relative tolerance the library's tests state.  Transposes and calcfl are
- * exact in both modes. */
+ * exact in both modes, and so is the calcfl that plans and ensembles fuse
+ * into the last sweep of a step (the next step's dt): in FAST it screens the
+ * cells in FP32 and evaluates the reference's bound, bitwise, on every cell
+ * that can hold the maximum.  Switching the mode of a plan or ensemble
+ * recomputes calcfl of its state before the next step. */
 #define MHD_PREC_EXACT 0
 #define MHD_PREC_FAST 1
 

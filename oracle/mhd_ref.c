@@ -33,6 +33,9 @@
  *   step_any          closed-form index formula + involution; equal to
  *                     transpose13 / the reference step where ny == nz;
  *                     conserved mass and energy, div.B at round-off
+ *   diagnostics       (no reference counterpart) sums against math.fsum;
+ *                     div.B against a numpy forward-difference divergence,
+ *                     zero for a forward curl, planted-face closed forms
  */
 #include <math.h>
 #include <stdint.h>
@@ -102,34 +105,6 @@ static real ref_flux(int n, const real *u, const real *bc, real *v)
         for (int i = 0; i < 5 * n; i++) v[i] = v[i] / c;
     }
     return c;
-}
-
-/* Per-cell pieces of ref_flux for unit checks: un-normalised flux v (5,n)
- * and each cell's freezing-speed candidate cf (n). */
-void mhdref_flux_cells(int n, const real *u, const real *bc, real *v, real *cf)
-{
-    for (int i = 0; i < n; i++) {
-        real tmp[5];
-        real c = ref_flux(1, u + 5 * i, bc + 3 * i, tmp);
-        /* ref_flux normalised tmp by c; undo by recomputing without it */
-        const real gam = (real)(5.0 / 3.0);
-        const real *q = u + 5 * i;
-        const real *bb = bc + 3 * i;
-        real rho = q[0], mx = q[1], my = q[2], mz = q[3], e = q[4];
-        real b1 = bb[0], b2 = bb[1], b3 = bb[2];
-        real vx = mx / rho;
-        real msq = mx * mx + my * my + mz * mz;
-        real bsq = b1 * b1 + b2 * b2 + b3 * b3;
-        real ps = (real)(((double)e - (double)(msq / rho) / 2.0) * ((double)gam - 1.0)
-                         + (2.0 - (double)gam) * (double)bsq / 2.0);
-        real *w = v + 5 * i;
-        w[0] = mx;
-        w[1] = mx * vx + ps - b1 * b1;
-        w[2] = my * vx - b2 * b1;
-        w[3] = mz * vx - b3 * b1;
-        w[4] = (e + ps) * vx - b1 * (b1 * mx + b2 * my + b3 * mz) / rho;
-        cf[i] = c;
-    }
 }
 
 /* limiter of C/tvd1.c:75-79: 0, or 2ab/(a+b) in double when a*b>0 (float test) */

@@ -83,6 +83,41 @@ def traj_case(name, nx, ny, nz, spec, nsteps, cfl, keep_final=False):
     print("wrote", f"traj_{name}.json", flush=True)
 
 
+def oracle_traj_case(name, n, spec, nsteps, cfl, threads=8):
+    """A trajectory too large for the compiled reference's static buffers
+    (dims <= 100): per-step SHA-256 and dt of the oracle's FP32 path, which
+    equals the compiled reference bitwise on every case above, plus the FP32
+    path's error against the FP64 oracle at the last step."""
+    from oracle import Oracle
+    u, b = solver.ic_host(n, n, n, spec)
+    rec = {"dims": [n, n, n], "spec": spec.__dict__, "nsteps": nsteps, "cfl": cfl,
+           "ic_sha_u": sha(u), "ic_sha_b": sha(b), "source": "oracle (pinned to the compiled reference)"}
+    steps = []
+
+    def on_step(s, uu, bb, dt):
+        steps.append({"dt": float(dt), "sha_u": sha(uu), "sha_b": sha(bb)})
+        if (s + 1) % 10 == 0:
+            print(f"  {name} f32 step {s + 1}", flush=True)
+
+    u32, b32, t32, _ = Oracle("f32", threads=threads).run(u, b, n, n, n, nsteps, cfl=cfl, on_step=on_step)
+    rec["f32"] = {"t": float(t32), "steps": steps}
+    u64, b64, t64, dts64 = Oracle("f64", threads=threads).run(u.astype(np.float64), b.astype(np.float64),
+                                                             n, n, n, nsteps, cfl=cfl)
+    g32 = {"rho": u32.reshape(-1, 5)[:, 0], "mom": u32.reshape(-1, 5)[:, 1:4], "E": u32.reshape(-1, 5)[:, 4],
+           "B": b32.reshape(-1, 3)}
+    g64 = {"rho": u64.reshape(-1, 5)[:, 0], "mom": u64.reshape(-1, 5)[:, 1:4], "E": u64.reshape(-1, 5)[:, 4],
+           "B": b64.reshape(-1, 3)}
+    err = {}
+    for k in g32:
+        d = np.linalg.norm(g32[k].astype(np.float64) - g64[k])
+        err[k] = {"rel_l2": float(d / np.linalg.norm(g64[k])),
+                  "rel_fluct_l2": float(d / np.linalg.norm(g64[k] - g64[k].mean(axis=0)))}
+    rec["f64"] = {"t": float(t64), "dt_last": float(dts64[-1])}
+    rec["f32_vs_f64"] = err
+    (HERE / f"traj_{name}.json").write_text(json.dumps(rec, indent=1))
+    print("wrote", f"traj_{name}.json", flush=True)
+
+
 def main():
     smooth = solver.ICSpec(seed=7, amp_v=0.3, amp_a=0.1)
     op_case("16x16x16", 16, 16, 16, smooth, 0.1)
@@ -96,7 +131,12 @@ def main():
     traj_case("32", 32, 32, 32, solver.ICSpec(seed=5, amp_v=0.3, amp_a=0.1), 10, 0.7)
     # the CPU-runnable headline case: 64^3, amplitudes 0.1 / 0.05, 10 steps, CFL 0.7
     traj_case("64_c1", 64, 64, 64, solver.ICSpec(seed=1, amp_v=0.1, amp_a=0.05), 10, 0.7)
+    # the single-box GPU workload: 256^3, same generator and parameters, 100 steps
+    oracle_traj_case("256_c2", 256, solver.ICSpec(seed=1, amp_v=0.1, amp_a=0.05), 100, 0.7)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["256_c2"]:
+        oracle_traj_case("256_c2", 256, solver.ICSpec(seed=1, amp_v=0.1, amp_a=0.05), 100, 0.7)
+    else:
+        main()

@@ -112,3 +112,41 @@ def test_ensemble_size_ranges_match_single_box_runs(gpu, precision):
         assert np.float32(t[i]) == np.float32(ref[i][2]) and np.float32(dt[i]) == np.float32(ref[i][3])
     ens.close()
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("precision", ["fast", "exact"])
+def test_ensemble_workload_96_boxes_50_steps(gpu, precision):
+    """The batched-ensemble workload as benchmarked: 96 boxes with sides
+    drawn by seed 7 from {32,48,64,96,128}, per-box amplitudes, 50 steps of
+    the autotuner's default shape, graph-replayed.  One box of each side
+    equals a single-box plan of the same box bitwise (state, t, dt); every
+    box keeps mass and energy to FP32 round-off and div b at round-off."""
+    from heterogeneous_mhd_amd import solver
+    sides, specs = solver.ensemble_specs(96, seed=7)
+    print(f"96-box ensemble ({precision}), {sum(n ** 3 for n in sides)} cells, 50 steps", flush=True)
+    ens = solver.Ensemble(sides, cfl=0.7, precision=precision)
+    ens.init(specs)
+    d0 = [ens.diagnostics(i) for i in range(len(sides))]
+    ens.run(50, graph=True)
+    t, dt = ens.times()
+    for i in range(len(sides)):
+        d1 = ens.diagnostics(i)
+        assert abs(d1["mass"] - d0[i]["mass"]) <= 1e-6 * abs(d0[i]["mass"]), f"mass of box {i}"
+        assert abs(d1["energy"] - d0[i]["energy"]) <= 1e-5 * abs(d0[i]["energy"]), f"energy of box {i}"
+        assert d1["divb_max"] <= 1e-5, f"div b of box {i}"
+    first = {}
+    for i, n in enumerate(sides):
+        first.setdefault(n, i)
+    assert sorted(first) == [32, 48, 64, 96, 128]
+    for n, i in sorted(first.items()):
+        print(f"  box {i} (side {n}) against a single-box plan", flush=True)
+        box = solver.Box(n, n, n, cfl=0.7, precision=precision)
+        box.init(specs[i])
+        box.run(50, graph=True)
+        bt, bdt = box.time()
+        bu, bb = box.host_state()
+        box.close()
+        eu, eb = ens.box_state(i)
+        assert bits_equal(bu, eu) and bits_equal(bb, eb), f"box {i} side {n}"
+        assert np.float32(t[i]) == np.float32(bt) and np.float32(dt[i]) == np.float32(bdt), f"box {i}"
+    ens.close()

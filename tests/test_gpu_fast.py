@@ -139,3 +139,74 @@ def test_fast_ensemble_boxes_match_single_box_fast_runs(gpu):
         assert bits_equal(bu, eu) and bits_equal(bb, eb), f"box {i} side {n}"
     ens.close()
     torch.cuda.synchronize()
+
+
+def _dt_of(u, b, n, cfl=0.7):
+    """dt the reference's driver takes from this state (C/mhd.c:62-63)."""
+    from oracle import Oracle
+    return np.float32(cfl * float(Oracle("f32", threads=16).calcfl(u, b, n, n, n)))
+
+
+def _check_fused_cfl(u0, b0, n, steps):
+    """Step a fast-mode plan one step at a time: every step's dt, which from
+    the second step on comes from the cfl the previous step's last sweep
+    fused, equals the reference calcfl of the state it starts from."""
+    import torch
+    from heterogeneous_mhd_amd import solver
+    box = solver.Box(n, n, n, cfl=0.7, precision="fast")
+    box.set_state(torch.from_numpy(u0), torch.from_numpy(b0))
+    u, b = u0, b0
+    for s in range(steps):
+        want = _dt_of(u, b, n)
+        box.run(1, graph=s > 0)
+        got = np.float32(box.time()[1])
+        assert got.tobytes() == want.tobytes(), f"step {s}: dt {got!r} vs calcfl {want!r}"
+        u, b = box.host_state()
+    box.close()
+
+
+def test_fast_fused_cfl_is_the_exact_calcfl(gpu, ic):
+    u0, b0 = ic
+    print("fast mode, 64^3: fused cfl against the reference calcfl, 6 steps", flush=True)
+    _check_fused_cfl(u0, b0, N, 6)
+
+
+def test_fast_fused_cfl_exact_on_unscreened_and_near_uniform_states(gpu):
+    """Two states that take the other branches of the screened maximum: a
+    cell outside the screened domain (huge energy) makes its workgroup
+    evaluate every cell exactly; a nearly uniform state puts every cell
+    within the screen's keep band."""
+    from heterogeneous_mhd_amd import solver
+    n = 32
+    u, b = solver.ic_host(n, n, n, solver.ICSpec(seed=3, amp_v=0.1, amp_a=0.05))
+    u = u.copy()
+    u[(((5 * n) + 7) * n + 9) * 5 + 4] = np.float32(3.0e9)
+    print("fast mode, 32^3 with a huge cell: fused cfl", flush=True)
+    _check_fused_cfl(u, b, n, 3)
+    uu, bu = solver.ic_host(n, n, n, solver.ICSpec(seed=5, amp_v=1e-6, amp_a=1e-6))
+    print("fast mode, 32^3 nearly uniform: fused cfl", flush=True)
+    _check_fused_cfl(uu, bu, n, 3)
+
+
+def test_fast_to_exact_switch_recomputes_calcfl(gpu, ic):
+    """A plan switched from fast to exact without a new state takes its next
+    dt from calcfl of the state it holds, and its step is bitwise the
+    reference's step from that state."""
+    import torch
+    from heterogeneous_mhd_amd import solver
+    from oracle import Oracle
+    u0, b0 = ic
+    box = solver.Box(N, N, N, cfl=0.7, precision="fast")
+    box.set_state(torch.from_numpy(u0), torch.from_numpy(b0))
+    box.run(2, graph=True)
+    uf, bf = box.host_state()
+    t_fast = np.float32(box.time()[0])
+    box.set_precision("exact")
+    box.run(1, graph=True)
+    u, b = box.host_state()
+    t, dt = box.time()
+    box.close()
+    ref = Oracle("f32", threads=16).run(uf, bf, N, N, N, 1, t=float(t_fast))
+    assert np.float32(dt).tobytes() == ref[3][0].tobytes()
+    assert np.float32(t).tobytes() == np.float32(ref[2]).tobytes()
+    assert bits_equal(u, ref[0]) and bits_equal(b, ref[1])

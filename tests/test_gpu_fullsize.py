@@ -135,3 +135,134 @@ def test_step_1024_invariants(gpu, big_state):
     assert abs(d1["mass"] - d0["mass"]) <= 1e-6 * abs(d0["mass"])
     assert abs(d1["energy"] - d0["energy"]) <= 1e-6 * abs(d0["energy"])
     assert d1["divb_max"] <= 1e-5
+
+
+# ---- the tolerance (fast) mode at the headline workload ---------------------
+# Stated tolerances (relative to the field group's largest magnitude in the
+# compared cells; groups rho, momentum, E, B):
+#  * one sweep, on sampled pencils: L-inf <= 2e-6 against the reference FP32
+#    path (the oracle), and L-inf <= 2e-6 against the FP64 build;
+#  * 20 full steps of the whole box: L-inf <= 1e-2 and L2 <= 1e-4 against the
+#    exact mode, which is bitwise the reference FP32 path.
+
+def _group_err(x_u, x_b, r_u, r_b):
+    """{group: L-inf error relative to the reference group's largest magnitude}."""
+    xu, ru = np.asarray(x_u, np.float64).reshape(-1, 5), np.asarray(r_u, np.float64).reshape(-1, 5)
+    xb, rb = np.asarray(x_b, np.float64).reshape(-1, 3), np.asarray(r_b, np.float64).reshape(-1, 3)
+    out = {}
+    for name, a, r in (("rho", xu[:, 0], ru[:, 0]), ("mom", xu[:, 1:4], ru[:, 1:4]),
+                       ("E", xu[:, 4], ru[:, 4]), ("B", xb, rb)):
+        if np.abs(r).max() > 0:
+            out[name] = float(np.abs(a - r).max() / np.abs(r).max())
+    return out
+
+
+FAST_OPS = ["fluidx", "advectbyzx", "advect_fluidx", "advect2_fluidx"]
+
+
+@pytest.mark.parametrize("op", FAST_OPS)
+def test_fast_sweeps_1024_sampled_pencils_within_tolerance(gpu, big_state, op):
+    import torch
+    from heterogeneous_mhd_amd import ops
+    from oracle import Oracle
+    u, b = big_state
+    print(f"fast {op} 1024^3 (v 0.3)", flush=True)
+    uo = torch.empty_like(u) if op != "advectbyzx" else None
+    bo = b.clone() if op != "fluidx" else None
+    if op == "fluidx":
+        ops.fluidx(uo, b, N, N, N, float(DT), u, b, precision="fast")
+    elif op == "advectbyzx":
+        ops.advectbyzx(u, bo, N, N, N, float(DT), u, b, precision="fast")
+    else:
+        getattr(ops, op)(uo, bo, N, N, N, float(DT), u, b, precision="fast")
+    torch.cuda.synchronize()
+    o32, o64 = Oracle("f32"), Oracle("f64")
+    worst32, worst64, worst_ref64 = {}, {}, {}
+    for j, k in _samples():
+        rows = [(j + dj, k + dk) for dk in (-1, 0, 1) for dj in (-1, 0, 1)]
+        ub = np.concatenate(_pencils(u, 5, N, rows))
+        bb = np.concatenate(_pencils(b, 3, N, rows))
+        c = 1 * 3 + 1  # the centre pencil (j, k) of the 3x3 box
+        refs = []
+        for o, cast in ((o32, np.float32), (o64, np.float64)):
+            ui, bi, dt = ub.astype(cast), bb.astype(cast), cast(DT)
+            ru, rb = ui, bi
+            if op == "fluidx":
+                ru = o.fluidx(ui, bi, N, 3, 3, dt)
+            elif op == "advectbyzx":
+                rb = o.advectbyzx(ui, bi, N, 3, 3, dt)
+            else:
+                rb = o.advectbyzx(ui, bi, N, 3, 3, dt)
+                if op == "advect2_fluidx":
+                    rb = o.advectbyzx(ui, rb, N, 3, 3, dt)
+                ru = o.fluidx(ui, rb, N, 3, 3, dt)
+            refs.append((ru[c * 5 * N:(c + 1) * 5 * N], rb[c * 3 * N:(c + 1) * 3 * N]))
+        gu = _pencils(uo, 5, N, [(j, k)])[0] if uo is not None else refs[0][0]
+        gb = _pencils(bo, 3, N, [(j, k)])[0] if bo is not None else refs[0][1]
+        for acc, ref in ((worst32, refs[0]), (worst64, refs[1])):
+            for g, e in _group_err(gu, gb, *ref).items():
+                acc[g] = max(acc.get(g, 0.0), e)
+        for g, e in _group_err(refs[0][0], refs[0][1], *refs[1]).items():
+            worst_ref64[g] = max(worst_ref64.get(g, 0.0), e)
+    print(f"{op}: fast vs f32 {worst32}\n  fast vs f64 {worst64}\n  f32 vs f64 {worst_ref64}", flush=True)
+    for g in worst32:
+        assert worst32[g] <= 2e-6, (g, worst32[g])
+        assert worst64[g] <= 2e-6, (g, worst64[g])
+    del uo, bo
+
+
+def test_fast_plan_1024_20_steps_tolerance_invariants_and_exact_cfl(gpu, big_state):
+    import torch
+    from heterogeneous_mhd_amd import ops, solver
+    u, b = big_state
+    box = solver.Box(N, N, N, cfl=0.7)
+    print("exact plan, 20 steps, 1024^3 (v 0.3)", flush=True)
+    box.set_state(u, b)
+    d0 = box.diagnostics()
+    box.run(20, graph=True)
+    box.sync()
+    ue, be = box.u.clone(), box.b.clone()
+    te = box.time()
+    print("fast plan, 20 steps, 1024^3 (v 0.3)", flush=True)
+    box.set_precision("fast")
+    box.set_state(u, b)
+    box.run(20, graph=True)
+    box.sync()
+    tf = box.time()
+    d1 = box.diagnostics()
+    # the next step's dt comes from the cfl fused into step 20's last sweep:
+    # it must be the exact calcfl of the state (the stand-alone kernel is
+    # bitwise the reference's calcfl)
+    inv_c = ops.calcfl(box.u, box.b, N, N, N)
+    errs = {}
+    uf = box.u.view(-1, 5)
+    uev = ue.view(-1, 5)
+    for name, cs in (("rho", [0]), ("mom", [1, 2, 3]), ("E", [4])):
+        num_inf, den_inf, num2, den2 = 0.0, 0.0, 0.0, 0.0
+        for c in cs:
+            d = (uf[:, c] - uev[:, c])
+            num_inf = max(num_inf, float(d.abs().max()))
+            den_inf = max(den_inf, float(uev[:, c].abs().max()))
+            num2 += float(torch.linalg.vector_norm(d, dtype=torch.float64)) ** 2
+            den2 += float(torch.linalg.vector_norm(uev[:, c], dtype=torch.float64)) ** 2
+            del d
+        errs[name] = (num_inf / den_inf, (num2 / den2) ** 0.5)
+    d = box.b - be
+    errs["B"] = (float(d.abs().max()) / float(be.abs().max()),
+                 float(torch.linalg.vector_norm(d, dtype=torch.float64))
+                 / float(torch.linalg.vector_norm(be, dtype=torch.float64)))
+    del d, ue, be
+    box.run(1, graph=False)
+    t21, dt21 = box.time()
+    box.close()
+    del box
+    torch.cuda.empty_cache()
+    print(f"fast vs exact after 20 steps (L-inf, L2): {errs}\n t exact {te} fast {tf}\n"
+          f" diag {d0} -> {d1}", flush=True)
+    for g, (linf, l2) in errs.items():
+        assert linf <= 1e-2 and l2 <= 1e-4, (g, linf, l2)
+    assert abs(d1["mass"] - d0["mass"]) <= 1e-6 * abs(d0["mass"])
+    assert abs(d1["energy"] - d0["energy"]) <= 1e-5 * abs(d0["energy"])
+    assert d1["divb_max"] <= 1e-5
+    want = np.float32(0.7 * float(np.float32(inv_c)))
+    assert np.float32(dt21).tobytes() == want.tobytes(), (dt21, want)
