@@ -4,8 +4,8 @@ Workload (default): one 1024^3 periodic FP32 box, seeded random-phase IC with
 velocity amplitude 0.3 and vector-potential amplitude 0.05, CFL 0.7, 20
 full steps.  A step is one full x,y,z,z,y,x split step (the reference's 16
 operators: calcfl, 6 fluid sweeps, 6 CT field sweeps, 4 transposes; the
-three advectbyzx+fluidx pairs of the second half run as one fused kernel
-each) replayed from a captured HIP graph; state, t and dt stay in HBM.
+first half's two advectbyzx+transpose pairs and the second half's three
+advectbyzx+fluidx pairs run as one fused kernel each) replayed from a captured HIP graph; state, t and dt stay in HBM.
 `value` is measured in the FP32 tolerance mode (precision fast, checked
 against the reference by tests/test_gpu_fast.py); the bitwise-exact mode is
 timed beside it ("exact").  Prints one JSON line:
@@ -44,12 +44,14 @@ import numpy as np
 PEAK_HBM_GBS = 8000.0
 # compulsory bytes per cell per call at each operator's C-call boundary (FP32)
 ALGO_BYTES = {"fluidx": 52, "advectbyzx": 32, "transpose12": 64, "transpose13": 64, "calcfl": 32,
-              "advect_fluidx": 64, "advect2_fluidx": 64}
-# launches per step of the plan: the second half's three advectbyzx+fluidx
-# pairs run fused, the first of them together with the first half's last
-# advectbyzx (advect2_fluidx), and calcfl is fused into the last pair
-CALLS_PER_STEP = {"fluidx": 3, "advectbyzx": 2, "transpose12": 2, "transpose13": 2,
-                  "advect_fluidx": 2, "advect2_fluidx": 1}
+              "advect_fluidx": 64, "advect2_fluidx": 64, "advect_transpose12": 64,
+              "advect_transpose13": 64}
+# launches per step of the plan: the first half's two advectbyzx+transpose
+# pairs run fused, the second half's three advectbyzx+fluidx pairs run fused,
+# the first of them together with the first half's last advectbyzx
+# (advect2_fluidx), and calcfl is fused into the last pair
+CALLS_PER_STEP = {"fluidx": 3, "advect_transpose12": 1, "advect_transpose13": 1, "transpose12": 1,
+                  "transpose13": 1, "advect_fluidx": 2, "advect2_fluidx": 1}
 # the reference decomposition the step roofline is priced on (792 B/cell)
 REF_CALLS = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13": 2, "calcfl": 1}
 # the kernel (as rocprofv3 names it) each operator launches, and that
@@ -59,11 +61,14 @@ KERNEL_OF = {"fluidx": "fluidx_kernel", "advectbyzx": "advect_kernel",
              "advect_fluidx": "advect_fluid_kernel (1 pass)",
              "advect2_fluidx": "advect_fluid_kernel (2 passes)",
              "transpose12": "transpose_kernel", "transpose13": "transpose_kernel",
-             "calcfl": "cfl_kernel"}
+             "advect_transpose12": "advect_transpose_kernel (12)",
+             "advect_transpose13": "advect_transpose_kernel (13)", "calcfl": "cfl_kernel"}
 TRAFFIC_KEY = {"fluidx_kernel": "fluidx", "advect_kernel": "advectbyzx",
                "advect_fluid_kernel (1 pass)": "advect_fluidx",
                "advect_fluid_kernel (2 passes)": "advect2_fluidx",
-               "transpose_kernel": "transpose (u and b)", "cfl_kernel": "calcfl"}
+               "transpose_kernel": "transpose (u and b)", "cfl_kernel": "calcfl",
+               "advect_transpose_kernel (12)": "advect_transpose12",
+               "advect_transpose_kernel (13)": "advect_transpose13"}
 PRECISION_INDEPENDENT = {"transpose_kernel", "cfl_kernel"}
 METRIC = "cell-updates/s & % HBM roofline per 3D step"
 
@@ -83,6 +88,10 @@ def kernel_times(box, reps: int, precision: str = "exact"):
         "advect2_fluidx": lambda: ops.advect2_fluidx(u2, b2, nx, ny, nz, dt, u, b, precision=precision),
         "transpose12": lambda: ops.transpose12(u2, b2, u, b, nx, ny, nz),
         "transpose13": lambda: ops.transpose13(u2, b2, u, b, nx, ny, nz),
+        "advect_transpose12": lambda: ops.advect_transpose12(u2, b2, nx, ny, nz, dt, u, b,
+                                                             precision=precision),
+        "advect_transpose13": lambda: ops.advect_transpose13(u2, b2, nx, ny, nz, dt, u, b,
+                                                             precision=precision),
         "calcfl": lambda: ops.calcfl_device(u, b, nx, ny, nz),
     }
     out = {}

@@ -308,6 +308,24 @@ int mhd_advect2_fluidx_async(float *u, float *b, const float *update_u, const fl
                                     precision == MHD_PREC_FAST, 2);
 }
 
+int mhd_advect_transpose12_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream)
+{
+    if (precision != MHD_PREC_EXACT && precision != MHD_PREC_FAST) return MHD_ERR_ARG;
+    return mhd::launch_advect_transpose(u, b, update_u, update_b, nx, ny, nz, dt_dev, dt, (hipStream_t)stream,
+                                        precision == MHD_PREC_FAST, 12);
+}
+
+int mhd_advect_transpose13_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream)
+{
+    if (precision != MHD_PREC_EXACT && precision != MHD_PREC_FAST) return MHD_ERR_ARG;
+    return mhd::launch_advect_transpose(u, b, update_u, update_b, nx, ny, nz, dt_dev, dt, (hipStream_t)stream,
+                                        precision == MHD_PREC_FAST, 13);
+}
+
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream)
 {
@@ -460,14 +478,16 @@ int mhd_plan_set_precision(mhd_plan *p, int mode)
 
 // One step, C/mhd.c:59-124.  Operators map state -> state; the reference's
 // copy_matrix calls disappear into a rotation over (U0,B0), (U1,B1) and a
-// third face-field buffer B2 of the plan.  The three advectbyzx + fluidx
-// pairs of the second half run fused, and the middle of the step -- the
-// first half's last advectbyzx and the second half's first pair, whose two
-// advectbyzx calls use the same u -- is one kernel whose intermediate field
-// never leaves the registers.  The y and z fluid sweeps of the first half
-// work in place; every other operator writes another buffer than it reads.
-// u changes buffer 8 times and b 9 times per step; with three b buffers the
-// state still ends where it started.  calcfl (:62) of the next step is fused
+// third face-field buffer B2 of the plan.  The first half's two
+// advectbyzx + transpose pairs run fused (the new field is written only
+// transposed), the three advectbyzx + fluidx pairs of the second half run
+// fused, and the middle of the step -- the first half's last advectbyzx and
+// the second half's first pair, whose two advectbyzx calls use the same u --
+// is one kernel whose intermediate field never leaves the registers.  The y
+// and z fluid sweeps of the first half work in place; every other operator
+// writes another buffer than it reads.  u changes buffer 8 times and b 7
+// times per step; with three b buffers the state still ends where it
+// started.  calcfl (:62) of the next step is fused
 // into this step's last fluid sweep, which writes 1/c and the unclamped dt
 // to scal[2..3]; the step begins by applying the clamp and advancing t
 // (:63-64).
@@ -486,12 +506,11 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
     RUN(mhd::launch_begin_step(t, dt, dtn, p->tf, s));                                  // :63-64
     // x sweep (out of place, so that the fused second half ends in U0)
     RUN(mhd::launch_fluidx(U1, U0, B0, nx, ny, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :71
-    RUN(mhd::launch_advectbyzx(B1, B0, U1, B0, nx, ny, nz, dt, 0.0f, s, f));               // :74
-    RUN(mhd::launch_transpose12(U0, B0, U1, B1, nx, ny, nz, s));                         // :78
+    // :74 advectbyzx and :78 transpose12 in one kernel (the new b is written transposed only)
+    RUN(mhd::launch_advect_transpose(U0, B1, U1, B0, nx, ny, nz, dt, 0.0f, s, f, 12));     // :74, :78
     // y sweep, dims (ny, nx, nz)
-    RUN(mhd::launch_fluidx(U0, U0, B0, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :80
-    RUN(mhd::launch_advectbyzx(B1, B0, U0, B0, ny, nx, nz, dt, 0.0f, s, f));               // :84
-    RUN(mhd::launch_transpose13(U1, B0, U0, B1, ny, nx, nz, s, true));                   // :88
+    RUN(mhd::launch_fluidx(U0, U0, B1, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :80
+    RUN(mhd::launch_advect_transpose(U1, B0, U0, B1, ny, nx, nz, dt, 0.0f, s, f, 13));     // :84, :88
     // z sweeps, dims (nz, nx, ny)
     RUN(mhd::launch_fluidx(U1, U1, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :91
     // :93 advectbyzx and the second half's first pair (:95-97) in one kernel

@@ -111,6 +111,13 @@ int launch_advect_fluid(float *u_out, float *b_out, const float *uu, const float
                         int nz, const float *dt_dev, float dt_val, hipStream_t s, void *cfl_ws,
                         float *inv_c, float *dt_next, double cfl, bool fast, int npass = 1);
 
+// advectbyzx (b_main = bu) followed by transpose12 (which = 12, out dims
+// (n2, n1, n3)) or the general transpose13 (which = 13, out dims (n3, n2,
+// n1)) of uu and the new b, in one kernel (mhd_advect_transpose.hip); bitwise
+// equal to the two launches.  No output may overlap an input.
+int launch_advect_transpose(float *u_out, float *b_out, const float *uu, const float *bu, int n1, int n2, int n3,
+                            const float *dt_dev, float dt_val, hipStream_t s, bool fast, int which);
+
 int launch_transpose12(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                        int nz, hipStream_t s);
 

@@ -92,6 +92,31 @@ def advect2_fluidx(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
                                          _stream()), "advect2_fluidx")
 
 
+def advect_transpose12(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
+    """advectbyzx then transpose12 as one kernel (bitwise equal to the two
+    calls, C/mhd.c:74, :78): u, b <- transpose12(update_u, CT update of
+    update_b), dims (ny, nx, nz).  No output may overlap an input."""
+    n = nx * ny * nz
+    dp, dv = _dt(dt)
+    check(lib().mhd_advect_transpose12_async(_dev(u, 5 * n, "u"), _dev(b, 3 * n, "b"),
+                                             _dev(update_u, 5 * n, "update_u"),
+                                             _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv,
+                                             MHD_PREC_FAST if precision == "fast" else MHD_PREC_EXACT,
+                                             _stream()), "advect_transpose12")
+
+
+def advect_transpose13(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
+    """advectbyzx then the general transpose13 as one kernel (C/mhd.c:84,
+    :88): out dims (nz, ny, nx), any nx, nz."""
+    n = nx * ny * nz
+    dp, dv = _dt(dt)
+    check(lib().mhd_advect_transpose13_async(_dev(u, 5 * n, "u"), _dev(b, 3 * n, "b"),
+                                             _dev(update_u, 5 * n, "update_u"),
+                                             _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv,
+                                             MHD_PREC_FAST if precision == "fast" else MHD_PREC_EXACT,
+                                             _stream()), "advect_transpose13")
+
+
 def transpose12(u, b, update_u, update_b, nx, ny, nz, iter=0):
     """x<->y swap of update_* into u, b (C/transpose12.c:7-47)."""
     del iter

@@ -38,6 +38,11 @@ _OPS = [
     (re.compile(r"\bfluidx_kernel<\d+, \d+, true, true[,>]"), "fluidx+calcfl fast"),
     (re.compile(r"\badvect_kernel<\d+, \d+, false[,>]"), "advectbyzx"),
     (re.compile(r"\badvect_kernel<\d+, \d+, true[,>]"), "advectbyzx fast"),
+    (re.compile(r"\badvect_transpose_kernel<\d+, false, 12>"), "advect_transpose12"),
+    (re.compile(r"\badvect_transpose_kernel<\d+, true, 12>"), "advect_transpose12 fast"),
+    (re.compile(r"\badvect_transpose_kernel<\d+, false, 13>"), "advect_transpose13"),
+    (re.compile(r"\badvect_transpose_kernel<\d+, true, 13>"), "advect_transpose13 fast"),
+    (re.compile(r"\bstream_copy_kernel"), "copy_matrix (streaming copy)"),
     (re.compile(r"\btranspose_kernel(?!<)"), "transpose (u and b)"),
     (re.compile(r"\btranspose_kernel<5>"), "transpose (u, 5 comps)"),
     (re.compile(r"\btranspose_kernel<3>"), "transpose (b, 3 comps)"),

@@ -127,6 +127,20 @@ int mhd_advect_fluidx_async(float *u, float *b, const float *update_u, const flo
 int mhd_advect2_fluidx_async(float *u, float *b, const float *update_u, const float *update_b,
                              int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
                              mhd_stream_t stream);
+/* advectbyzx (the new b from update_b, with b_main = update_b) followed by
+ * transpose12 of update_u and that b, in one kernel, bitwise equal to the
+ * two calls -- C/mhd.c:74 and :78 (no reference counterpart as one call).
+ * u, b receive the transposed state, dims (ny, nx, nz); the new b in the
+ * input layout is not written.  No output may overlap an input.
+ * precision: MHD_PREC_EXACT or MHD_PREC_FAST. */
+int mhd_advect_transpose12_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream);
+/* The same with transpose13 for any nx, nz (out dims (nz, ny, nx), the
+ * permutation of mhd_transpose13_any_async) -- C/mhd.c:84 and :88. */
+int mhd_advect_transpose13_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream);
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream);
 int mhd_transpose13_async(float *u, float *b, const float *update_u, const float *update_b, int nx,

@@ -270,3 +270,58 @@ def test_advect2_fluidx_equals_the_three_calls(gpu, shape, precision):
         ob = O.advectbyzx(u, O.advectbyzx(u, b, nx, ny, nz, dt), nx, ny, nz, dt)
         assert bits_equal(_h(bf), ob)
         assert bits_equal(_h(uf), O.fluidx(u, ob, nx, ny, nz, dt))
+
+
+# tiles of 56 cells x 16 pencils: full and partial tiles along both axes,
+# pencils shorter than the 64-cell window (several wraps), partner extents
+# below 16, a remaining axis of extent 1 (the walk's look-ahead row wraps
+# twice), non-cubic boxes
+AT_SHAPES = [(16, 16, 16), (24, 16, 16), (12, 10, 10), (7, 3, 7), (1, 4, 4), (3, 1, 2), (2, 1, 1), (5, 2, 1),
+             (64, 48, 40),
+             (100, 33, 17), (57, 16, 20), (130, 20, 3), (56, 32, 16), (112, 17, 5), (1024, 32, 3),
+             (256, 5, 33)]
+
+
+@pytest.mark.parametrize("shape", AT_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("which", [12, 13])
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_advect_transpose_equals_the_two_calls(gpu, shape, which, precision):
+    """advect_transpose12/13 is bitwise advectbyzx followed by transpose12
+    (resp. the general transpose13): C/mhd.c:74, :78 and :84, :88; in exact
+    mode also the oracle's two calls."""
+    import torch
+    from heterogeneous_mhd_amd import ops
+    from oracle import Oracle
+    nx, ny, nz = shape
+    print(f"advect_transpose{which} {nx}x{ny}x{nz} {precision}", flush=True)
+    u, b = _random_state(nx, ny, nz, seed=nx + 3 * ny + nz)
+    dt = np.float32(0.05)
+    ud, bd = _t(u, gpu), _t(b, gpu)
+    b1 = bd.clone()
+    ops.advectbyzx(ud, b1, nx, ny, nz, float(dt), ud, bd, precision=precision)
+    tu, tb = torch.empty_like(ud), torch.empty_like(bd)
+    if which == 12:
+        ops.transpose12(tu, tb, ud, b1, nx, ny, nz)
+    else:
+        ops.transpose13_any(tu, tb, ud, b1, nx, ny, nz)
+    fu, fb = torch.full_like(ud, np.nan), torch.full_like(bd, np.nan)
+    getattr(ops, f"advect_transpose{which}")(fu, fb, nx, ny, nz, float(dt), ud, bd, precision=precision)
+    assert bits_equal(_h(fb), _h(tb))
+    assert bits_equal(_h(fu), _h(tu))
+    if precision == "exact":
+        O = Oracle("f32")
+        ob = O.advectbyzx(u, b, nx, ny, nz, dt)
+        ru, rb = O.transpose12(u, ob, nx, ny, nz) if which == 12 else O.transpose13_any(u, ob, nx, ny, nz)
+        assert bits_equal(_h(fu), ru) and bits_equal(_h(fb), rb)
+
+
+def test_advect_transpose_rejects_overlap(gpu):
+    import torch
+    from heterogeneous_mhd_amd import MHDError, ops
+    n = 8
+    u, b = _random_state(n, n, n, seed=1)
+    ud, bd = _t(u, gpu), _t(b, gpu)
+    with pytest.raises(MHDError):
+        ops.advect_transpose12(ud, torch.empty_like(bd), n, n, n, 0.05, ud, bd)
+    with pytest.raises(MHDError):
+        ops.advect_transpose13(torch.empty_like(ud), bd, n, n, n, 0.05, ud, bd)
