@@ -25,12 +25,14 @@ MHD_ERR_HIP = -3
 MHD_ERR_TOO_LONG = -4
 MHD_ERR_NOMEM = -5
 MHD_ERR_IO = -6
+MHD_ERR_SCRATCH = -7
 MHD_PREC_EXACT = 0
 MHD_PREC_FAST = 1
 
 _ERRS = {MHD_ERR_ARG: "bad argument", MHD_ERR_SHAPE: "unsupported shape",
          MHD_ERR_HIP: "HIP runtime error", MHD_ERR_TOO_LONG: "pencil too long",
-         MHD_ERR_NOMEM: "out of device memory", MHD_ERR_IO: "file error"}
+         MHD_ERR_NOMEM: "out of device memory", MHD_ERR_IO: "file error",
+         MHD_ERR_SCRATCH: "box exceeds the fixed scratch"}
 
 # every symbol include/mhd_hip.h declares, with (restype, argtypes)
 _fp = C.POINTER(C.c_float)
@@ -70,6 +72,8 @@ SIGNATURES = {
     "mhd_calcfl_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "mhd_calcfl_async": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_double, _vp,
                                    _vp, C.c_float, _vp]),
+    "mhd_calcfl_noalloc_async": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, _vp,
+                                           _vp, C.c_float, _vp]),
     "mhd_plan_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_double, C.c_float,
                                   _vp]),
     "mhd_plan_destroy": (None, [_vp]),

@@ -368,6 +368,27 @@ int mhd_calcfl_async(float *inv_c_dev, const float *u, const float *b, int nx, i
                               (hipStream_t)stream);
 }
 
+int mhd_calcfl_noalloc_async(float *inv_c_dev, const float *u, const float *b, int nx, int ny, int nz,
+                             double cfl, float *dt_dev, float *t_dev, float tf, mhd_stream_t stream)
+{
+    if (!inv_c_dev || !u || !b || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
+    const size_t ncell = (size_t)nx * ny * nz;
+    if (ncell > mhd::noalloc_max_cells()) {
+        g_err = "calcfl_noalloc: " + std::to_string(ncell) + " cells exceed the fixed scratch (at most " +
+                std::to_string(mhd::noalloc_max_cells()) + " cells)";
+        return MHD_ERR_SCRATCH;
+    }
+    hipError_t e = hipSuccess;
+    void *ws = mhd::noalloc_scratch(&e);
+    if (!ws) {
+        note(e, "calcfl_noalloc: scratch address");
+        return MHD_ERR_HIP;
+    }
+    // the fold's ticket counter is cleared by a kernel first, so the call
+    // does not rely on what an earlier call left in the shared array
+    return mhd::launch_calcfl(inv_c_dev, u, b, nx, ny, nz, ws, cfl, dt_dev, t_dev, tf, (hipStream_t)stream, true);
+}
+
 // ---- device-side step driver ---------------------------------------------
 
 struct mhd_plan {

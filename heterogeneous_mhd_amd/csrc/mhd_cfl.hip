@@ -9,6 +9,8 @@
 // kernel reduces the records in order (two levels, the second by the last
 // workgroup to finish).  Per-cell arithmetic follows the reference's
 // float/double pattern, so 1/c is bit-exact.
+#include <mutex>
+
 #include "mhd_cfl_common.h"
 
 namespace mhd {
@@ -110,6 +112,14 @@ __global__ __launch_bounds__(kFoldBlock) void cfl_fold_kernel(int nrec, CflOut o
     }
 }
 
+// zeroes the workspace header (the fold's ticket counter) with vector
+// stores: a kernel, not a memset node, so that a captured graph replays it
+__global__ void cfl_clear_kernel(void *ws)
+{
+    ((unsigned *)ws)[threadIdx.x] = 0u;  // 64 x 4 bytes = kCflHeader
+}
+static_assert(kCflHeader == 64 * sizeof(unsigned), "one thread per header word");
+
 }  // namespace
 
 static_assert(kFoldGroups == kFoldBlock, "level 2 folds one record per thread");
@@ -137,17 +147,59 @@ size_t cfl_workspace_bytes(int nx, int ny, int nz)
 }
 
 int launch_calcfl(float *inv_c, const float *u, const float *b, int nx, int ny, int nz, void *ws,
-                  double cfl, float *dt_dev, float *t_dev, float tf, hipStream_t s)
+                  double cfl, float *dt_dev, float *t_dev, float tf, hipStream_t s, bool clear_ticket)
 {
     if (!inv_c || !u || !b || !ws || nx < 1 || ny < 1 || nz < 1) return MHD_ERR_ARG;
     const size_t ncell = (size_t)nx * ny * nz;
     const size_t nrec = (ncell + kRun - 1) / kRun;
     if (nrec > 0x7fffffff) return MHD_ERR_ARG;
     (void)hipGetLastError();  // clear a stale error so the check below sees this launch
+    if (clear_ticket) hipLaunchKernelGGL(cfl_clear_kernel, dim3(1), dim3(64), 0, s, ws);
     hipLaunchKernelGGL(cfl_kernel, dim3((unsigned)nrec), dim3(kCB), 0, s, u, b, ncell, ws);
     if (hipGetLastError() != hipSuccess) return MHD_ERR_HIP;
     CflOut o = {ws, inv_c, dt_dev, t_dev, cfl, tf};
     return launch_cfl_fold((int)nrec, o, s);
+}
+
+// ---- calcfl with no allocation ----------------------------------------------
+// For callers that cannot allocate (code under HIP graph capture, serving
+// loops): the workspace of launch_calcfl is a file-scope device array of a
+// fixed size, one copy per GPU, which the library holds until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device
+// memory: its device address comes from hipGetSymbolAddress, once per
+// device with that device current, and only that address is used.
+
+namespace {
+
+constexpr size_t kNoAllocMaxCells = (size_t)1 << 30;  // 1024^3
+constexpr size_t kScratchBytes =
+    kCflHeader + (kFoldGroups + (kNoAllocMaxCells + kRun - 1) / kRun) * sizeof(CflRec);  // 8 MiB + 2304 B
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+void *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+}  // namespace
+
+size_t noalloc_scratch_bytes() { return kScratchBytes; }
+size_t noalloc_max_cells() { return kNoAllocMaxCells; }
+
+// The current device's scratch address (nullptr and the HIP error on failure).
+void *noalloc_scratch(hipError_t *err)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        *err = e != hipSuccess ? e : hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::call_once(g_scratch_once[dev], [dev] {
+        g_scratch_err[dev] = hipGetSymbolAddress(&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+    });
+    *err = g_scratch_err[dev];
+    return g_scratch_err[dev] == hipSuccess ? g_scratch_ptr[dev] : nullptr;
 }
 
 }  // namespace mhd

@@ -137,8 +137,18 @@ size_t fluid_workgroups(int nx, long npen);
 // dt_dev is non-null also applies the driver's dt rule on the device:
 //   dt = (float)(cfl * (double)inv_c); if (tf > 0 && dt > (tf - t)/2) dt = (tf - t)/2;
 //   t = t + 2*dt     (t_dev may be null: then no clamp and no t update)
+// clear_ticket: zero the workspace header first (a kernel), for a workspace
+// whose counter is not known to be zero.
 int launch_calcfl(float *inv_c, const float *u, const float *b, int nx, int ny, int nz, void *ws,
-                  double cfl, float *dt_dev, float *t_dev, float tf, hipStream_t s);
+                  double cfl, float *dt_dev, float *t_dev, float tf, hipStream_t s,
+                  bool clear_ticket = false);
+
+// The no-allocation calcfl's fixed workspace (mhd_cfl.hip): its size, the
+// largest box it serves, and its address on the current device (nullptr, with
+// *err set, if the lookup failed).
+size_t noalloc_scratch_bytes();
+size_t noalloc_max_cells();
+void *noalloc_scratch(hipError_t *err);
 
 // copy_matrix on the device (mhd_copy.hip): dst0 <- src0 (n0 floats) and
 // dst1 <- src1 (n1 floats) in one streaming launch.

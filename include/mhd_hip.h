@@ -72,6 +72,7 @@ float calcfl(float *u, float *b, int *nx, int *ny, int *nz);
 #define MHD_ERR_TOO_LONG -4  /* pencil longer than mhd_max_pencil() */
 #define MHD_ERR_NOMEM -5     /* device allocation failed */
 #define MHD_ERR_IO -6        /* a file could not be opened, read or written */
+#define MHD_ERR_SCRATCH -7   /* the box exceeds the fixed scratch of mhd_calcfl_noalloc_async */
 
 typedef void *mhd_stream_t; /* a hipStream_t; NULL is the null stream */
 
@@ -164,6 +165,18 @@ size_t mhd_calcfl_workspace_bytes(int nx, int ny, int nz);
 int mhd_calcfl_async(float *inv_c_dev, const float *u, const float *b, int nx, int ny, int nz,
                      void *workspace, double cfl, float *dt_dev, float *t_dev, float tf,
                      mhd_stream_t stream);
+/* calcfl that allocates nothing (no counterpart in the reference): for
+ * callers that cannot allocate -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same results and
+ * arguments as mhd_calcfl_async without the workspace: the library uses a
+ * fixed device array of 8 MiB + 2304 bytes per GPU (held until the process
+ * ends), which serves boxes of up to 2^30 cells (1024^3); larger boxes return
+ * MHD_ERR_SCRATCH with a message in mhd_last_error().  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (two streams, two
+ * threads) race; calls ordered on one stream do not.  Calls no hipMalloc,
+ * hipFree or allocating library routine. */
+int mhd_calcfl_noalloc_async(float *inv_c_dev, const float *u, const float *b, int nx, int ny, int nz,
+                             double cfl, float *dt_dev, float *t_dev, float tf, mhd_stream_t stream);
 
 /* ---- full step on the device (the sweep order of C/mhd.c:59-124) ---- */
 typedef struct mhd_plan mhd_plan;
