@@ -304,8 +304,8 @@ def main():
     ap.add_argument("--no-exact", action="store_true", help="skip the bitwise-exact leg")
     ap.add_argument("--no-corrector", action="store_true")
     ap.add_argument("--no-sizes", action="store_true", help="skip the 64^3 / 256^3 throughputs")
-    ap.add_argument("--corrector-seeds", type=int, default=2)
-    ap.add_argument("--corrector-steps", type=int, default=100)
+    ap.add_argument("--corrector-seeds", type=int, default=8)
+    ap.add_argument("--corrector-steps", type=int, default=200)
     ap.add_argument("--precision", choices=["exact", "fast"], default="fast")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="print only the 96-box ensemble measurement (for profiling it)")

@@ -12,7 +12,9 @@ potential, so div b stays at round-off.  The relaxation alpha (and an
 optional limit on the output values) is chosen on validation seeds by
 `select` and stored with the weights.  The
 correction is skipped (the plain step is kept) when the corrected state has
-rho <= 0 or p <= 0 anywhere or changes the total energy by more than etol.
+rho <= 0 or p <= 0 anywhere, or when it converts more than etol of the box's
+total energy between heat and kinetic plus magnetic energy (the total of E
+itself cannot change: the correction of u has zero box total).
 
 Training pairs come from the solver itself: a 256^3 run of a seeded IC is
 downsampled by 4 (block means of u, face means of b); one plain coarse step

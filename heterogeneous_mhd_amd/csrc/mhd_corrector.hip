@@ -651,48 +651,60 @@ __global__ void apply_kernel(const float *__restrict__ u, const float *__restric
     for (int d = 0; d < 3; d++) bo[v * 3 + d] = b[v * 3 + d] + scale[5] * cb[d];
 }
 
-// Per block: sums of E of the state and of the candidate (double), and
-// whether every candidate cell has rho > 0 and p > 0 (p from E, m and the
-// cell-centred B, gamma = 5/3).
-__global__ __launch_bounds__(256) void check_kernel(const float *__restrict__ u,
+// Per block (double): the kinetic plus magnetic energy of the state and of
+// the candidate, the state's total energy E, and whether every candidate
+// cell has rho > 0 and p > 0 (p from E, m and the cell-centred B, gamma =
+// 5/3).  The correction leaves the box total of E unchanged (its u part has
+// zero box mean), so the change of kinetic plus magnetic energy is the
+// thermal energy it converts; the decision bounds that.
+__device__ __forceinline__ double kin_mag(const float *q, const float *b, long v, int i, int j, int k, int nx,
+                                          int ny, int nz)
+{
+    const float b1 = 0.5f * (b[v * 3] + b[vid(wrap(i + 1, nx), j, k, nx, ny) * 3]);
+    const float b2 = 0.5f * (b[v * 3 + 1] + b[vid(i, wrap(j + 1, ny), k, nx, ny) * 3 + 1]);
+    const float b3 = 0.5f * (b[v * 3 + 2] + b[vid(i, j, wrap(k + 1, nz), nx, ny) * 3 + 2]);
+    const double kin = 0.5 * ((double)q[1] * q[1] + (double)q[2] * q[2] + (double)q[3] * q[3]) / (double)q[0];
+    return kin + 0.5 * ((double)b1 * b1 + (double)b2 * b2 + (double)b3 * b3);
+}
+
+__global__ __launch_bounds__(256) void check_kernel(const float *__restrict__ u, const float *__restrict__ b,
                                                     const float *__restrict__ uc,
                                                     const float *__restrict__ bc, int nx, int ny,
                                                     int nz, double *__restrict__ part)
 {
-    __shared__ double e0[256], e1[256];
+    __shared__ double e0[256], e1[256], et[256];
     __shared__ int okb[256];
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double a = 0.0, c = 0.0;
+    double a = 0.0, c = 0.0, e = 0.0;
     int ok = 1;
     if (v < (long)nx * ny * nz) {
         const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
-        a = u[v * 5 + 4];
-        const float *q = uc + v * 5;
-        c = q[4];
-        const float b1 = 0.5f * (bc[v * 3] + bc[vid(wrap(i + 1, nx), j, k, nx, ny) * 3]);
-        const float b2 = 0.5f * (bc[v * 3 + 1] + bc[vid(i, wrap(j + 1, ny), k, nx, ny) * 3 + 1]);
-        const float b3 = 0.5f * (bc[v * 3 + 2] + bc[vid(i, j, wrap(k + 1, nz), nx, ny) * 3 + 2]);
-        const double rho = q[0];
-        const double kin = 0.5 * ((double)q[1] * q[1] + (double)q[2] * q[2] + (double)q[3] * q[3]) / rho;
-        const double p = (2.0 / 3.0) * ((double)q[4] - kin - 0.5 * ((double)b1 * b1 + (double)b2 * b2 + (double)b3 * b3));
+        a = kin_mag(u + v * 5, b, v, i, j, k, nx, ny, nz);
+        c = kin_mag(uc + v * 5, bc, v, i, j, k, nx, ny, nz);
+        e = u[v * 5 + 4];
+        const double rho = uc[v * 5];
+        const double p = (2.0 / 3.0) * ((double)uc[v * 5 + 4] - c);
         ok = rho > 0.0 && p > 0.0;  // false for NaN too
     }
     e0[threadIdx.x] = a;
     e1[threadIdx.x] = c;
+    et[threadIdx.x] = e;
     okb[threadIdx.x] = ok;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
             e0[threadIdx.x] += e0[threadIdx.x + w];
             e1[threadIdx.x] += e1[threadIdx.x + w];
+            et[threadIdx.x] += et[threadIdx.x + w];
             okb[threadIdx.x] &= okb[threadIdx.x + w];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = e0[0];
-        part[3 * blockIdx.x + 1] = e1[0];
-        part[3 * blockIdx.x + 2] = okb[0];
+        part[4 * blockIdx.x] = e0[0];
+        part[4 * blockIdx.x + 1] = e1[0];
+        part[4 * blockIdx.x + 2] = et[0];
+        part[4 * blockIdx.x + 3] = okb[0];
     }
 }
 
@@ -702,29 +714,32 @@ __global__ __launch_bounds__(256) void decide_kernel(const double *__restrict__ 
                                                      double etol, int *__restrict__ flag,
                                                      int *__restrict__ stats)
 {
-    __shared__ double e0[256], e1[256];
+    __shared__ double e0[256], e1[256], et[256];
     __shared__ int okb[256];
-    double a = 0.0, c = 0.0;
+    double a = 0.0, c = 0.0, e = 0.0;
     int ok = 1;
     for (int q = threadIdx.x; q < nblk; q += 256) {
-        a += part[3 * q];
-        c += part[3 * q + 1];
-        ok &= part[3 * q + 2] != 0.0;
+        a += part[4 * q];
+        c += part[4 * q + 1];
+        e += part[4 * q + 2];
+        ok &= part[4 * q + 3] != 0.0;
     }
     e0[threadIdx.x] = a;
     e1[threadIdx.x] = c;
+    et[threadIdx.x] = e;
     okb[threadIdx.x] = ok;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
             e0[threadIdx.x] += e0[threadIdx.x + w];
             e1[threadIdx.x] += e1[threadIdx.x + w];
+            et[threadIdx.x] += et[threadIdx.x + w];
             okb[threadIdx.x] &= okb[threadIdx.x + w];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const bool acc = okb[0] && fabs(e1[0] - e0[0]) <= etol * fabs(e0[0]);
+        const bool acc = okb[0] && fabs(e1[0] - e0[0]) <= etol * fabs(et[0]);
         flag[0] = acc;
         stats[acc ? 0 : 1] += 1;
     }
@@ -895,14 +910,14 @@ int mhd_corr_apply(float *u, float *b, const float *y, int nx, int ny, int nz, c
     const long n = (long)nx * ny * nz;
     const unsigned nb = blocks(n, 256);
     const hipStream_t s = (hipStream_t)stream;
-    double *part = (double *)check_ws;  // 5 doubles per block, reused by the check
+    double *part = (double *)check_ws;  // 5 doubles per block, reused (4 per block) by the check
     float *ymean = (float *)(part + 5 * (size_t)nb);
     (void)hipGetLastError();
     hipLaunchKernelGGL(ysum_kernel, dim3(nb), dim3(256), 0, s, y, n, limit, part);
     hipLaunchKernelGGL(ymean_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, n, ymean);
     hipLaunchKernelGGL(apply_kernel, dim3(nb), dim3(256), 0, s, u, b, y, nx, ny, nz, scale, ymean,
                        limit, u_cand, b_cand);
-    hipLaunchKernelGGL(check_kernel, dim3(nb), dim3(256), 0, s, u, u_cand, b_cand, nx, ny, nz, part);
+    hipLaunchKernelGGL(check_kernel, dim3(nb), dim3(256), 0, s, u, b, u_cand, b_cand, nx, ny, nz, part);
     hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, etol, flag_stats,
                        flag_stats + 1);
     hipLaunchKernelGGL(commit_kernel, dim3(nb), dim3(256), 0, s, u, b, u_cand, b_cand, n, flag_stats);

@@ -350,8 +350,11 @@ int mhd_downsample(const float *u, const float *b, int nx, int ny, int nz, int r
 /* Applies the correction y (FP32 [V][32] output layer) to (u, b) -- each
  * output value limited to [-limit, limit] when limit > 0, the u part
  * projected to zero box total, as the scheme conserves the totals of rho, m
- * and E -- unless the corrected state fails a check: rho > 0 and p > 0 in every cell and
- * |sum E - sum E_before| <= etol * |sum E_before| (FP64 sums).  u_cand,
+ * and E -- unless the corrected state fails a check: rho > 0 and p > 0 in
+ * every cell, and the box total of kinetic plus magnetic energy (cell-centred
+ * B) moves by at most etol times the box total of E, i.e. the correction
+ * converts at most that share of the energy to or from heat (FP64 sums; the
+ * total of E itself is unchanged by the zero-total projection).  u_cand,
  * b_cand: scratch of the state's sizes; check_ws: mhd_corr_check_bytes();
  * flag_stats: 3 device ints: [0] = this call accepted, [1] += accepted,
  * [2] += rejected.  Entirely on the device (graph-capturable). */

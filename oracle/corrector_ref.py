@@ -176,20 +176,28 @@ def loss_and_grad(y, target):
     return loss, g
 
 
+def kinetic_magnetic(u, b):
+    """Per cell: m^2 / (2 rho) + |B_c|^2 / 2 with the cell-centred B."""
+    f = features(u, b)
+    return 0.5 * np.sum(f[..., 1:4] ** 2, axis=-1) / f[..., 0] + 0.5 * np.sum(f[..., 5:8] ** 2, axis=-1)
+
+
 def apply_correction(u, b, y, scale, etol, limit=0.0, gamma=5.0 / 3.0):
     """(u', b', accepted): the corrected state if it passes the positivity
     and energy checks, else the input state.  Each output value is limited to
     [-limit, limit] (limit > 0); the u part of the correction is projected to
-    zero box total (the scheme conserves the totals of u)."""
+    zero box total (the scheme conserves the totals of u), so the total of E
+    does not move and the energy check bounds the kinetic plus magnetic
+    energy the correction converts to or from heat: |delta sum(K + M)| <=
+    etol |sum E|."""
     if limit > 0:
         y = np.clip(y, -limit, limit)
     yu = y[..., :5] - y[..., :5].mean(axis=(0, 1, 2))
     uc = u + scale[:5] * yu
     bcand = b + scale[5] * curl(y[..., 5:8])
-    f = features(uc, bcand)
-    rho = f[..., 0]
-    p = (gamma - 1.0) * (f[..., 4] - 0.5 * np.sum(f[..., 1:4] ** 2, axis=-1) / rho
-                         - 0.5 * np.sum(f[..., 5:8] ** 2, axis=-1))
-    e0, e1 = u[..., 4].sum(), uc[..., 4].sum()
-    ok = bool(np.all(rho > 0) and np.all(p > 0) and abs(e1 - e0) <= etol * abs(e0))
+    km1 = kinetic_magnetic(uc, bcand)
+    rho = uc[..., 0]
+    p = (gamma - 1.0) * (uc[..., 4] - km1)
+    e0, e1 = kinetic_magnetic(u, b).sum(), km1.sum()
+    ok = bool(np.all(rho > 0) and np.all(p > 0) and abs(e1 - e0) <= etol * abs(u[..., 4].sum()))
     return (uc, bcand, True) if ok else (u, b, False)

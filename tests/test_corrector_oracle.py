@@ -137,3 +137,17 @@ def test_apply_correction_checks():
     y[..., 0] = 0.0
     y[1, 2, 3, 4] = -5.0  # p < 0 in one cell
     assert not R.apply_correction(u, b, y, scale, etol=1e-3)[2]
+    # a momentum correction that turns 0.5% of the energy into kinetic energy:
+    # positivity holds, the total of E does not move, the energy check refuses it
+    y[...] = 0.0
+    y[..., 1] = 0.1 * np.where(np.arange(4) % 2 == 0, 1.0, -1.0)[None, None, :]
+    uu, _, ok = R.apply_correction(u, b, y, scale, etol=1e-3)
+    kin = 0.5 * 0.1 ** 2  # per cell, rho = 1, E = 1
+    assert not ok and uu is u
+    assert R.apply_correction(u, b, y, scale, etol=1.1 * kin)[2]
+    assert not R.apply_correction(u, b, y, scale, etol=0.9 * kin)[2]
+    # a field correction converts magnetic energy likewise
+    y[...] = 0.0
+    y[..., 7] = 0.2 * np.sin(2 * np.pi * np.arange(4) / 4)[None, :, None]
+    _, bb, ok = R.apply_correction(u, b, y, scale, etol=1e-4)
+    assert not ok and not bb.any()

@@ -251,12 +251,14 @@ def test_apply_accepts_and_falls_back(gpu):
     y = (0.01 * RNG.standard_normal((nz, ny, nx, 32))).astype(np.float32)
     stats = torch.zeros(3, dtype=torch.int32, device="cuda")
     scale = np.array([0.5] * 5 + [0.2])
-    for case, etol in (("accept", 1e-2), ("negative p", 1e-2), ("negative rho", 1e-2)):
+    for case, etol in (("accept", 1e-2), ("negative p", 1e-2), ("negative rho", 1e-2), ("energy", 1e-2)):
         yy = y.copy()
         if case == "negative rho":
             yy[1, 2, 3, 0] = -10.0
         if case == "negative p":
             yy[2, 1, 7, 4] = -10.0
+        if case == "energy":  # kinetic energy 0.5 * (0.5 * 0.4)^2 = 2% of E = 2 per cell
+            yy[..., 1] = 0.4 * np.where(np.arange(nx) % 2 == 0, 1.0, -1.0)
         ud, bd, yd = _dev(u.reshape(-1)), _dev(b.reshape(-1)), _dev(yy.reshape(-1))
         net.apply(buf, ud, bd, yd, etol, stats)
         ur, br, ok = R.apply_correction(u.astype(np.float64), b.astype(np.float64),
@@ -266,7 +268,7 @@ def test_apply_accepts_and_falls_back(gpu):
         ug = ud.cpu().numpy().reshape(nz, ny, nx, 5)
         bg = bd.cpu().numpy().reshape(nz, ny, nx, 3)
         assert np.allclose(ug, ur, rtol=1e-6, atol=1e-7) and np.allclose(bg, br, rtol=1e-6, atol=1e-7)
-    assert stats.cpu().tolist()[1:] == [1, 2]
+    assert stats.cpu().tolist()[1:] == [1, 3]
     # with the output limit active
     ud, bd, yd = _dev(u.reshape(-1)), _dev(b.reshape(-1)), _dev(y.reshape(-1))
     net.apply(buf, ud, bd, yd, 1e-2, stats, limit=0.005)
