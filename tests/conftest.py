@@ -40,6 +40,19 @@ def load_traj(name):
     return json.loads((GOLDEN / f"traj_{name}.json").read_text())
 
 
+@pytest.fixture(autouse=True)
+def _live_gpu_output(request):
+    """GPU tests print their progress as they run instead of at their end, so a
+    long one (the FP64 oracle of a 256^3 run takes minutes on the host) keeps
+    writing to the terminal.  Tests that read captured output keep capture."""
+    if request.node.get_closest_marker("gpu") is None or "capfd" in request.fixturenames:
+        yield
+        return
+    capman = request.config.pluginmanager.getplugin("capturemanager")
+    with capman.global_and_fixture_disabled():
+        yield
+
+
 @pytest.fixture(scope="session")
 def gpu():
     import torch
