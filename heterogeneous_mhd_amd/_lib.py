@@ -125,6 +125,8 @@ SIGNATURES = {
     "mhd_corr_adam": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_float, C.c_float, C.c_float,
                                 C.c_float, C.c_int, C.c_float, _vp]),
     "mhd_corr_pack": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "mhd_corr_pack_features": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp,
+                                         _vp]),
     "mhd_corr_loss_partials": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "mhd_corr_loss_grad": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "mhd_downsample": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),

@@ -80,21 +80,32 @@ class Buffers:
                                  device=device)
 
 
+# network input channels per input kind (mhd_corr_pack_features): 0 the 8
+# normalised fields; 1 plus their Laplacians; 2 plus their second differences
+# along x, y and z; 3 plus the low parts their bf16 rounding drops
+INPUT_CHANNELS = {0: 8, 1: 16, 2: 32, 3: 16}
+DERIVED = {0: 0, 1: 8, 2: 24, 3: 0}
+
+
 class CorrectorNet:
     """Parameters, optimizer state and the forward/backward/apply passes."""
 
-    def __init__(self, device="cuda", seed: int = 0):
+    def __init__(self, device="cuda", seed: int = 0, inputs: int = 0):
         self.device = torch.device(device)
+        if inputs not in INPUT_CHANNELS:
+            raise MHDError(f"unknown input kind {inputs}")
+        self.inputs = inputs
+        cin = INPUT_CHANNELS[inputs]
         self.np = lib().mhd_corr_num_params()
         nw = self.np - CH
         rng = np.random.default_rng(seed)
         p = np.zeros((LAYERS, self.np), np.float32)
         for li in range(LAYERS):
-            fan_in = 27 * (8 if li == 0 else CH)
+            fan_in = 27 * (cin if li == 0 else CH)
             std = np.sqrt(2.0 / fan_in) if li < LAYERS - 1 else 0.01 / np.sqrt(fan_in)
             w = rng.standard_normal((27, CH, CH)) * std
             if li == 0:
-                w[:, :, 8:] = 0.0          # padded input channels
+                w[:, :, cin:] = 0.0        # padded input channels
             if li == LAYERS - 1:
                 w[:, 8:, :] = 0.0          # padded output channels
             p[li, :nw] = w.reshape(-1)
@@ -108,11 +119,12 @@ class CorrectorNet:
         # the first layer's 8-channel input in the packed form (4 taps per K-step)
         self.wf0 = torch.empty(lib().mhd_corr_first_weight_elems(), dtype=torch.int16,
                                device=self.device)
-        # input normalisation (means, inverse std devs) and output scales
-        self.norm = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0] + [1] * 8, dtype=torch.float32,
+        # input normalisation (means, inverse std devs, inverse std devs of
+        # the derived channels) and output scales
+        self.norm = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0] + [1] * 32, dtype=torch.float32,
                                  device=self.device)
         self.scale = torch.ones(6, dtype=torch.float32, device=self.device)
-        self.meta: dict = {}
+        self.meta: dict = {"inputs": inputs}
         self.prep()
 
     # -- parameters ----------------------------------------------------------
@@ -133,11 +145,14 @@ class CorrectorNet:
     @classmethod
     def load(cls, path=WEIGHTS, device="cuda"):
         z = np.load(path, allow_pickle=False)
-        net = cls(device)
+        meta = json.loads(str(z["meta"]))
+        net = cls(device, inputs=int(meta.get("inputs", 0)))
         net.params.copy_(torch.from_numpy(z["params"]))
-        net.norm.copy_(torch.from_numpy(z["norm"]))
+        norm = torch.from_numpy(z["norm"])
+        net.norm[:norm.numel()].copy_(norm)
         net.scale.copy_(torch.from_numpy(z["scale"]))
-        net.meta = json.loads(str(z["meta"]))
+        net.meta = meta
+        net.meta["inputs"] = net.inputs
         net.prep()
         return net
 
@@ -146,11 +161,16 @@ class CorrectorNet:
         """Network output (FP32 [V][32]) for state (u, b)."""
         nx, ny, nz = buf.shape
         L, s = lib(), _st()
-        check(L.mhd_corr_pack(_p(u), _p(b), nx, ny, nz, _p(self.norm), _p(buf.x0), s), "corr_pack")
+        if self.inputs == 0:
+            check(L.mhd_corr_pack(_p(u), _p(b), nx, ny, nz, _p(self.norm), _p(buf.x0), s), "corr_pack")
+        else:
+            check(L.mhd_corr_pack_features(_p(u), _p(b), nx, ny, nz, _p(self.norm), self.inputs,
+                                           _p(buf.x0), s), "corr_pack_features")
         a = buf.x0
+        first = (3, self.wf0) if self.inputs == 0 else (0, self.wf[0])  # 8 channels: packed taps
         for li in range(LAYERS):
             last = li == LAYERS - 1
-            mode, w = (1, self.wf[li]) if last else (3, self.wf0) if li == 0 else (0, self.wf[li])
+            mode, w = (1, self.wf[li]) if last else first if li == 0 else (0, self.wf[li])
             check(L.mhd_corr_conv(_p(a), _p(w), _p(self.bias(li)), nx, ny, nz, mode,
                                   None if buf.z[li] is None else _p(buf.z[li]),
                                   None if last else _p(buf.a[li]),
@@ -316,13 +336,26 @@ def make_onpolicy_pairs(net: CorrectorNet, seeds, rollout: int, every: int, fine
     return pairs
 
 
+def second_differences(f, n):
+    """[V][8] normalised fields -> [V][8][3]: (f(-e_d) + f(+e_d)) - 2 f along
+    d = x, y, z (periodic), as mhd_corr_pack_features forms them."""
+    g = f.view(n, n, n, 8)
+    return torch.stack([(g.roll(1, a) + g.roll(-1, a)) - 2.0 * g for a in (2, 1, 0)],
+                       dim=-1).view(-1, 8, 3)
+
+
 def set_normalisation(net: CorrectorNet, pairs, n: int):
     f = torch.cat([features(u, b, n) for u, b, _ in pairs])
     mean, std = f.mean(0), f.std(0).clamp_min(1e-12)
     t = torch.cat([tg for _, _, tg in pairs])
     su = t[:, :5].std(0).clamp_min(1e-12)
     sb = t[:, 5:].std().clamp_min(1e-12)
-    net.norm.copy_(torch.cat([mean, 1.0 / std]))
+    net.norm[:16].copy_(torch.cat([mean, 1.0 / std]))
+    if DERIVED[net.inputs]:
+        sd = torch.cat([second_differences((features(u, b, n) - mean) / std, n)
+                        for u, b, _ in pairs[:64]])
+        d = sd.sum(-1) if net.inputs == 1 else sd.reshape(-1, 24)  # Laplacian / per axis
+        net.norm[16:16 + d.shape[1]].copy_(1.0 / d.std(0).clamp_min(1e-12))
     net.scale.copy_(torch.cat([su, sb.view(1)]))
 
 
@@ -610,6 +643,8 @@ def main(argv=None):
                    help="correction relaxations tried on validation seeds 4000.. after training")
     t.add_argument("--val-seeds", type=int, default=2)
     t.add_argument("--val-steps", type=int, default=100)
+    t.add_argument("--inputs", type=int, default=0, choices=sorted(INPUT_CHANNELS),
+                   help="network input kind (mhd_corr_pack_features)")
     f = sub.add_parser("finetune")
     f.add_argument("--init", default=str(WEIGHTS))
     f.add_argument("--seeds", type=int, default=100)
@@ -660,7 +695,7 @@ def main(argv=None):
         t0 = time.perf_counter()
         pairs = make_pairs(seeds, a.pairs, fine=a.fine, skip=(0.0, a.skip), log=log)
         t_data = time.perf_counter() - t0
-        net = CorrectorNet()
+        net = CorrectorNet(inputs=a.inputs)
         set_normalisation(net, pairs, n)
         t0 = time.perf_counter()
         hist = train(net, pairs, n, a.epochs, a.batch, a.lr, noise=a.noise, log=log)
@@ -673,7 +708,7 @@ def main(argv=None):
                             alpha=al, log=log)
             val[al] = summarize(rows)["err_mean"]["corrected"]
         alpha = min(val, key=val.get) if val else 1.0
-        net.meta = {"train_seeds": [seeds[0], seeds[-1]], "pairs_per_seed": a.pairs,
+        net.meta = {"inputs": a.inputs, "train_seeds": [seeds[0], seeds[-1]], "pairs_per_seed": a.pairs,
                     "skip_max": a.skip, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
                     "fine": a.fine, "coarse": n, "loss_history": hist, "data_s": t_data,
                     "train_s": t_train, "noise": a.noise, "alpha": alpha,

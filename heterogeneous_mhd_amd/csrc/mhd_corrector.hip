@@ -477,6 +477,73 @@ __global__ void pack_kernel(const float *__restrict__ u, const float *__restrict
         reinterpret_cast<uint4 *>(x + v * kC)[q] = reinterpret_cast<const uint4 *>(h)[q];
 }
 
+// The 8 normalised inputs at cell (i, j, k): (f - mean) * inv_std, FP32.
+__device__ __forceinline__ void norm_features(const float *__restrict__ u, const float *__restrict__ b,
+                                              int i, int j, int k, int nx, int ny, int nz,
+                                              const float *__restrict__ norm, float f[8])
+{
+    const long v = vid(i, j, k, nx, ny);
+#pragma unroll
+    for (int c = 0; c < 5; c++) f[c] = u[v * 5 + c];
+    f[5] = 0.5f * (b[v * 3 + 0] + b[vid(wrap(i + 1, nx), j, k, nx, ny) * 3 + 0]);
+    f[6] = 0.5f * (b[v * 3 + 1] + b[vid(i, wrap(j + 1, ny), k, nx, ny) * 3 + 1]);
+    f[7] = 0.5f * (b[v * 3 + 2] + b[vid(i, j, wrap(k + 1, nz), nx, ny) * 3 + 2]);
+#pragma unroll
+    for (int c = 0; c < 8; c++) f[c] = (f[c] - norm[c]) * norm[8 + c];
+}
+
+// Network input with derived channels (KIND 1: the 6-neighbour Laplacian of
+// each normalised input; 2: its second differences along x, y and z; 3: the
+// part of each normalised input that the bf16 rounding of channels 0..7
+// drops, times 256).  The derived values are formed in FP32 from the FP32
+// normalised inputs, so a bf16 channel carries a curvature to 8 significant
+// bits even where it is a small difference of O(1) values -- which the first
+// layer's bf16 products could not resolve from channels 0..7 alone.
+// Channel 8 + q holds derived value q times norm[16 + q].
+template <int KIND>
+__global__ void pack_features_kernel(const float *__restrict__ u, const float *__restrict__ b, int nx,
+                                     int ny, int nz, const float *__restrict__ norm,
+                                     u16 *__restrict__ x)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long)nx * ny * nz) return;
+    const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
+    float f0[8];
+    norm_features(u, b, i, j, k, nx, ny, nz, norm, f0);
+    u16 h[kC];
+#pragma unroll
+    for (int c = 0; c < 8; c++) h[c] = f2bf(f0[c]);
+#pragma unroll
+    for (int c = 8; c < kC; c++) h[c] = 0;
+    if constexpr (KIND == 3) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) h[8 + c] = f2bf((f0[c] - bf2f(h[c])) * 256.0f);
+    } else if constexpr (KIND == 1 || KIND == 2) {
+        float lap[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            float fm[8], fp[8];
+            norm_features(u, b, d == 0 ? wrap(i - 1, nx) : i, d == 1 ? wrap(j - 1, ny) : j,
+                          d == 2 ? wrap(k - 1, nz) : k, nx, ny, nz, norm, fm);
+            norm_features(u, b, d == 0 ? wrap(i + 1, nx) : i, d == 1 ? wrap(j + 1, ny) : j,
+                          d == 2 ? wrap(k + 1, nz) : k, nx, ny, nz, norm, fp);
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const float s = (fm[c] + fp[c]) - 2.0f * f0[c];
+                if constexpr (KIND == 1) lap[c] += s;
+                else h[8 + 3 * c + d] = f2bf(s * norm[16 + 3 * c + d]);
+            }
+        }
+        if constexpr (KIND == 1) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) h[8 + c] = f2bf(lap[c] * norm[16 + c]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        reinterpret_cast<uint4 *>(x + v * kC)[q] = reinterpret_cast<const uint4 *>(h)[q];
+}
+
 // b = curl A with A on edges (A1 at (i, j-1/2, k-1/2), ...), forward differences;
 // A is read from channels 5..7 of a [V][ld] array, each value limited to
 // [-lim, lim] when lim > 0.
@@ -863,6 +930,23 @@ int mhd_corr_pack(const float *u, const float *b, int nx, int ny, int nz, const 
     (void)hipGetLastError();
     hipLaunchKernelGGL(pack_kernel, dim3(blocks((long)nx * ny * nz, 256)), dim3(256), 0,
                        (hipStream_t)stream, u, b, nx, ny, nz, norm, x);
+    return done();
+}
+
+int mhd_corr_pack_features(const float *u, const float *b, int nx, int ny, int nz, const float *norm,
+                           int kind, uint16_t *x, mhd_stream_t stream)
+{
+    if (!u || !b || !norm || !x || nx < 1 || ny < 1 || nz < 1 || kind < 0 || kind > 3)
+        return MHD_ERR_ARG;
+    const dim3 g(blocks((long)nx * ny * nz, 256)), bs(256);
+    const hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    switch (kind) {
+    case 0: hipLaunchKernelGGL(pack_kernel, g, bs, 0, s, u, b, nx, ny, nz, norm, x); break;
+    case 1: hipLaunchKernelGGL(pack_features_kernel<1>, g, bs, 0, s, u, b, nx, ny, nz, norm, x); break;
+    case 2: hipLaunchKernelGGL(pack_features_kernel<2>, g, bs, 0, s, u, b, nx, ny, nz, norm, x); break;
+    default: hipLaunchKernelGGL(pack_features_kernel<3>, g, bs, 0, s, u, b, nx, ny, nz, norm, x); break;
+    }
     return done();
 }
 

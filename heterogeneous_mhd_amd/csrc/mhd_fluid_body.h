@@ -120,10 +120,13 @@ __device__ __forceinline__ T ld_at(const float *base, unsigned off)
 {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
 }
+// Output stores are nontemporal: a sweep's output is not read again by the
+// same launch, so it should not displace the neighbour pencils that later
+// workgroups of the launch read from L2.
 template <class T>
 __device__ __forceinline__ void st_at(float *base, unsigned off, T v)
 {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off));
 }
 
 // c*R + 0*R + 0 of the reference, see the file header.

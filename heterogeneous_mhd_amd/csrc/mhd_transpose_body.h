@@ -14,8 +14,10 @@
 // workgroup keeps its whole 20 KB (u) or 12 KB (b) tile in flight; the
 // component permutation is applied while scattering into LDS, which is laid
 // out in output order, so the store phase reads 16-byte LDS vectors and writes
-// 16-byte global vectors.  Partial tiles and unaligned shapes take a scalar
-// path.  Pure data movement: bit-exact by construction.
+// 16-byte global vectors, nontemporal (the output is not read again in the
+// launch; measured 1-2 ms faster per 1024^3 step).  Partial tiles and
+// unaligned shapes take a scalar path.  Pure data movement: bit-exact by
+// construction.
 #pragma once
 #include "mhd_internal.h"
 
@@ -72,8 +74,8 @@ __device__ __forceinline__ void tile_fast(float *__restrict__ out, const float *
     for (int k = 0; k < PER; k++) {
         const int f = threadIdx.x + k * kTB;
         const int i = f / R4, c4 = f - i * R4;
-        const float4 w = *reinterpret_cast<const float4 *>(tile + i * RS + 4 * c4);
-        *reinterpret_cast<float4 *>(out + (obase + (size_t)i * si_out) * NC + 4 * c4) = w;
+        const v4f w = *reinterpret_cast<const v4f *>(tile + i * RS + 4 * c4);
+        __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(out + (obase + (size_t)i * si_out) * NC + 4 * c4));
     }
 }
 

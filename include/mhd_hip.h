@@ -336,6 +336,16 @@ int mhd_corr_adam(float *p, float *g, float *m, float *v, int n, float lr, float
 /* State -> network input; norm[0..7] means, norm[8..15] inverse std devs. */
 int mhd_corr_pack(const float *u, const float *b, int nx, int ny, int nz, const float *norm,
                   uint16_t *x, mhd_stream_t stream);
+/* As mhd_corr_pack, plus derived channels formed in FP32 from the FP32
+ * normalised inputs f_c (channels 0..7) before their bf16 rounding:
+ *   kind 0: none (mhd_corr_pack);
+ *   kind 1: channel 8+c = (sum over the 6 neighbours of f_c - 6 f_c, as
+ *           the sum over d of (f_c(-e_d) + f_c(+e_d)) - 2 f_c) * norm[16+c];
+ *   kind 2: channel 8+3c+d = ((f_c(-e_d) + f_c(+e_d)) - 2 f_c) * norm[16+3c+d];
+ *   kind 3: channel 8+c = (f_c - bf16(f_c)) * 256.
+ * Periodic neighbours; norm holds 16 + (8 or 24) floats for kinds 1 and 2. */
+int mhd_corr_pack_features(const float *u, const float *b, int nx, int ny, int nz, const float *norm,
+                           int kind, uint16_t *x, mhd_stream_t stream);
 /* Loss = mean over voxels of |y_u - t_u|^2 + |curl(y_A) - t_B|^2 with
  * target [V][8] (normalised u correction, normalised b correction); writes
  * per-block loss sums (mhd_corr_loss_partials() doubles), the residual

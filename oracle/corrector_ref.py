@@ -120,6 +120,34 @@ def pack(u, b, mean, inv_std, bf=True):
     return x
 
 
+def pack_features(u, b, mean, inv_std, dnorm, kind):
+    """The network input of mhd_corr_pack_features, in FP32 in the kernel's
+    order: channels 0..7 bf16 of the normalised fields f; kind 1 adds the
+    Laplacian of f (channel 8+c), kind 2 its second differences along x, y
+    and z (channel 8+3c+d), each times dnorm; kind 3 adds (f - bf16(f)) * 256.
+    Returns float64 values of the bf16 channels [nz,ny,nx,32]."""
+    f32 = np.float32
+    u32, b32 = np.asarray(u, f32), np.asarray(b, f32)
+    bc = f32(0.5) * (b32 + np.stack([_shift(b32[..., 0], 0, 0, 1), _shift(b32[..., 1], 0, 1, 0),
+                                     _shift(b32[..., 2], 1, 0, 0)], axis=-1))
+    f = (np.concatenate([u32, bc], axis=-1) - np.asarray(mean, f32)) * np.asarray(inv_std, f32)
+    dn = np.asarray(dnorm, f32)
+    x = np.zeros(f.shape[:3] + (C,))
+    x[..., :8] = bf16(f)
+    if kind == 3:
+        x[..., 8:16] = bf16((f - bf16(f).astype(f32)) * f32(256.0))
+    elif kind in (1, 2):
+        steps = ((0, 0, 1), (0, 1, 0), (1, 0, 0))  # +e_x, +e_y, +e_z as (dz, dy, dx)
+        sd = [(_shift(f, -a, -bq, -c) + _shift(f, a, bq, c)) - f32(2.0) * f for a, bq, c in steps]
+        if kind == 1:
+            x[..., 8:16] = bf16((((f32(0.0) + sd[0]) + sd[1]) + sd[2]) * dn[:8])  # the kernel's sum
+        else:
+            for c in range(8):
+                for d in range(3):
+                    x[..., 8 + 3 * c + d] = bf16(sd[d][..., c] * dn[3 * c + d])
+    return x
+
+
 def downsample(u, b, r):
     """Block means by r: u over r^3 cells; b_d over the r^2 fine faces on the
     coarse cell's low d-face."""

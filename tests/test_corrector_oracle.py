@@ -151,3 +151,59 @@ def test_apply_correction_checks():
     y[..., 7] = 0.2 * np.sin(2 * np.pi * np.arange(4) / 4)[None, :, None]
     _, bb, ok = R.apply_correction(u, b, y, scale, etol=1e-4)
     assert not ok and not bb.any()
+
+
+def _random_state(nz=6, ny=5, nx=8):
+    u = RNG.standard_normal((nz, ny, nx, 5)).astype(np.float32)
+    u[..., 0] += 4.0
+    b = RNG.standard_normal((nz, ny, nx, 3)).astype(np.float32)
+    mean = RNG.standard_normal(8).astype(np.float32) * 0.1
+    inv = (1.0 + RNG.random(8)).astype(np.float32)
+    return u, b, mean, inv
+
+
+def _normalised(u, b, mean, inv):
+    # an independent FP64 statement of the normalised inputs
+    bc = 0.5 * (b + np.stack([np.roll(b[..., 0], -1, 2), np.roll(b[..., 1], -1, 1),
+                              np.roll(b[..., 2], -1, 0)], axis=-1))
+    return (np.concatenate([u, bc], axis=-1).astype(np.float64) - mean) * inv
+
+
+def test_pack_features_derived_channels_match_scipy():
+    """Laplacian and per-axis second differences against scipy.ndimage
+    (periodic), the low parts against the closed form hi + lo/256 = f, to the
+    bf16 rounding of the stored values."""
+    u, b, mean, inv = _random_state()
+    f = _normalised(u, b, mean, inv)
+    dn = (0.5 + RNG.random(24)).astype(np.float32)
+    x0 = R.pack_features(u, b, mean, inv, dn, 0)
+    np.testing.assert_allclose(x0[..., :8], f, rtol=2 ** -8, atol=1e-6)
+    assert not x0[..., 8:].any()
+    x1 = R.pack_features(u, b, mean, inv, dn, 1)
+    np.testing.assert_array_equal(x1[..., :8], x0[..., :8])
+    for c in range(8):
+        lap = ndimage.laplace(f[..., c], mode="wrap") * dn[c]
+        np.testing.assert_allclose(x1[..., 8 + c], lap, rtol=2 ** -7, atol=1e-5)
+    x2 = R.pack_features(u, b, mean, inv, dn, 2)
+    for c in range(8):
+        for d, axis in enumerate((2, 1, 0)):  # x, y, z
+            sd = ndimage.correlate1d(f[..., c], [1.0, -2.0, 1.0], axis=axis, mode="wrap") * dn[3 * c + d]
+            np.testing.assert_allclose(x2[..., 8 + 3 * c + d], sd, rtol=2 ** -7, atol=1e-5)
+    x3 = R.pack_features(u, b, mean, inv, dn, 3)
+    rec = x3[..., :8] + x3[..., 8:16] / 256.0
+    np.testing.assert_allclose(rec, f, rtol=2 ** -15, atol=1e-9)
+    # |f - bf16(f)| <= half a bf16 ulp <= 2^-8 |f|, so the scaled low part stays within |f|
+    assert np.all(np.abs(x3[..., 8:16]) <= np.abs(f) * (1 + 1e-6))
+
+
+def test_pack_features_axis_order():
+    """A field that varies along x only has zero y and z second differences:
+    a swapped axis (a plausible slip) fails here."""
+    nz, ny, nx = 4, 4, 8
+    u = np.zeros((nz, ny, nx, 5), np.float32)
+    u[..., 0] = 1.0 + 0.5 * np.sin(2 * np.pi * np.arange(nx) / nx)[None, None, :]
+    b = np.zeros((nz, ny, nx, 3), np.float32)
+    x2 = R.pack_features(u, b, np.zeros(8, np.float32), np.ones(8, np.float32),
+                         np.ones(24, np.float32), 2)
+    assert np.abs(x2[..., 8 + 0]).max() > 0.1        # rho along x
+    assert not x2[..., 8 + 1].any() and not x2[..., 8 + 2].any()

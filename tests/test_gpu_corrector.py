@@ -187,7 +187,7 @@ def test_network_gradients_match_oracle(gpu):
     b = 0.05 * RNG.standard_normal((nz, ny, nx, 3))
     tgt = RNG.standard_normal((nz, ny, nx, 8))
     norm = np.concatenate([[1.0, 0, 0, 0, 1.0, 0, 0, 0], [10.0] * 5 + [20.0] * 3]).astype(np.float32)
-    net.norm.copy_(torch.from_numpy(norm))
+    net.norm[:16].copy_(torch.from_numpy(norm))
     buf = corrector.Buffers(nx, ny, nz, net.device, train=True)
     ud, bd = _dev(u.astype(np.float32).reshape(-1)), _dev(b.astype(np.float32).reshape(-1))
     td = _dev(tgt.astype(np.float32).reshape(-1, 8))
@@ -217,6 +217,39 @@ def test_network_gradients_match_oracle(gpu):
         errb = np.linalg.norm(grads[li, 27 * 1024:] - og[li][1]) / np.linalg.norm(og[li][1])
         print(f"layer {li}: weight-gradient rel err {err:.2e}, bias {errb:.2e}", flush=True)
         assert err <= 5e-3 and errb <= 5e-3
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_pack_features_matches_oracle(gpu, kind):
+    """mhd_corr_pack_features against the oracle's FP32 restatement, bit for
+    bit (same operations, same order, no contraction); kind 0 is
+    mhd_corr_pack.  A non-cubic box so that a swapped axis shows."""
+    import torch
+    from heterogeneous_mhd_amd import corrector, lib
+    nx, ny, nz = 64, 6, 5
+    print(f"pack_features kind {kind} {nx}x{ny}x{nz}", flush=True)
+    u = np.zeros((nz, ny, nx, 5), np.float32)
+    u[..., 0] = 1.0 + 0.2 * RNG.standard_normal((nz, ny, nx))
+    u[..., 1:4] = 0.1 * RNG.standard_normal((nz, ny, nx, 3))
+    u[..., 4] = 1.5 + 0.2 * RNG.standard_normal((nz, ny, nx))
+    b = (0.05 * RNG.standard_normal((nz, ny, nx, 3))).astype(np.float32)
+    norm = np.concatenate([[1.0, 0, 0, 0, 1.5, 0, 0, 0], [5.0] * 5 + [20.0] * 3,
+                           0.5 + RNG.random(24)]).astype(np.float32)
+    ud, bd, nd = _dev(u.reshape(-1)), _dev(b.reshape(-1)), _dev(norm)
+    x = torch.empty(nx * ny * nz * 32, dtype=torch.int16, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    assert lib().mhd_corr_pack_features(ud.data_ptr(), bd.data_ptr(), nx, ny, nz, nd.data_ptr(), kind,
+                                        x.data_ptr(), st) == 0
+    want = R.pack_features(u, b, norm[:8], norm[8:16], norm[16:], kind)
+    got = _from_bits(x.cpu().numpy()).reshape(nz, ny, nx, 32)
+    np.testing.assert_array_equal(got, want)
+    if kind == 0:
+        x0 = torch.empty_like(x)
+        assert lib().mhd_corr_pack(ud.data_ptr(), bd.data_ptr(), nx, ny, nz, nd.data_ptr(), x0.data_ptr(),
+                                   st) == 0
+        assert torch.equal(x0, x)
+    assert lib().mhd_corr_pack_features(ud.data_ptr(), bd.data_ptr(), nx, ny, nz, nd.data_ptr(), 4,
+                                        x.data_ptr(), st) != 0
 
 
 def test_downsample_matches_oracle_and_keeps_divb(gpu):
