@@ -4,8 +4,10 @@ Workload (default): one 1024^3 periodic FP32 box, seeded random-phase IC with
 velocity amplitude 0.3 and vector-potential amplitude 0.05, CFL 0.7, 20
 full steps.  A step is one full x,y,z,z,y,x split step (the reference's 16
 operators: calcfl, 6 fluid sweeps, 6 CT field sweeps, 4 transposes; the
-first half's two advectbyzx+transpose pairs and the second half's three
-advectbyzx+fluidx pairs run as one fused kernel each) replayed from a captured HIP graph; state, t and dt stay in HBM.
+first half's two advectbyzx+transpose pairs, the second half's two
+transpose+advectbyzx pairs and the middle's advectbyzx, advectbyzx, fluidx
+run as one fused kernel each, and calcfl is fused into the last fluid sweep)
+replayed from a captured HIP graph; state, t and dt stay in HBM.
 `value` is measured in the FP32 tolerance mode (precision fast, checked
 against the reference by tests/test_gpu_fast.py); the bitwise-exact mode is
 timed beside it ("exact").  Prints one JSON line:
@@ -45,13 +47,24 @@ PEAK_HBM_GBS = 8000.0
 # compulsory bytes per cell per call at each operator's C-call boundary (FP32)
 ALGO_BYTES = {"fluidx": 52, "advectbyzx": 32, "transpose12": 64, "transpose13": 64, "calcfl": 32,
               "advect_fluidx": 64, "advect2_fluidx": 64, "advect_transpose12": 64,
-              "advect_transpose13": 64}
-# launches per step of the plan: the first half's two advectbyzx+transpose
-# pairs run fused, the second half's three advectbyzx+fluidx pairs run fused,
-# the first of them together with the first half's last advectbyzx
-# (advect2_fluidx), and calcfl is fused into the last pair
-CALLS_PER_STEP = {"fluidx": 3, "advect_transpose12": 1, "advect_transpose13": 1, "transpose12": 1,
-                  "transpose13": 1, "advect_fluidx": 2, "advect2_fluidx": 1}
+              "advect_transpose13": 64, "transpose12_advect": 64, "transpose13_advect": 64}
+
+
+def calls_per_step(n: int) -> dict:
+    """Launches per step of the plan on an n^3 box.  The first half's two
+    advectbyzx+transpose pairs run fused; the middle (the first half's last
+    advectbyzx and the second half's first advectbyzx+fluidx) is one kernel
+    (advect2_fluidx).  When the partner axis tiles into 16-pencil blocks (n %
+    16 == 0) the second half's two transpose+advectbyzx pairs run fused and
+    their fluid sweeps stand alone (the last also reduces calcfl, so its
+    launch takes somewhat longer than the plain fluidx timed here); otherwise
+    the transposes stand alone and each advectbyzx+fluidx pair is one kernel,
+    the last with calcfl."""
+    if n % 16 == 0:
+        return {"fluidx": 5, "advect_transpose12": 1, "advect_transpose13": 1, "advect2_fluidx": 1,
+                "transpose13_advect": 1, "transpose12_advect": 1}
+    return {"fluidx": 3, "advect_transpose12": 1, "advect_transpose13": 1, "transpose12": 1,
+            "transpose13": 1, "advect_fluidx": 2, "advect2_fluidx": 1}
 # the reference decomposition the step roofline is priced on (792 B/cell)
 REF_CALLS = {"fluidx": 6, "advectbyzx": 6, "transpose12": 2, "transpose13": 2, "calcfl": 1}
 # the kernel (as rocprofv3 names it) each operator launches, and that
@@ -62,13 +75,17 @@ KERNEL_OF = {"fluidx": "fluidx_kernel", "advectbyzx": "advect_kernel",
              "advect2_fluidx": "advect_fluid_kernel (2 passes)",
              "transpose12": "transpose_kernel", "transpose13": "transpose_kernel",
              "advect_transpose12": "advect_transpose_kernel (12)",
-             "advect_transpose13": "advect_transpose_kernel (13)", "calcfl": "cfl_kernel"}
+             "advect_transpose13": "advect_transpose_kernel (13)",
+             "transpose12_advect": "transpose_advect_kernel (12)",
+             "transpose13_advect": "transpose_advect_kernel (13)", "calcfl": "cfl_kernel"}
 TRAFFIC_KEY = {"fluidx_kernel": "fluidx", "advect_kernel": "advectbyzx",
                "advect_fluid_kernel (1 pass)": "advect_fluidx",
                "advect_fluid_kernel (2 passes)": "advect2_fluidx",
                "transpose_kernel": "transpose (u and b)", "cfl_kernel": "calcfl",
                "advect_transpose_kernel (12)": "advect_transpose12",
-               "advect_transpose_kernel (13)": "advect_transpose13"}
+               "advect_transpose_kernel (13)": "advect_transpose13",
+               "transpose_advect_kernel (12)": "transpose12_advect",
+               "transpose_advect_kernel (13)": "transpose13_advect"}
 PRECISION_INDEPENDENT = {"transpose_kernel", "cfl_kernel"}
 METRIC = "cell-updates/s & % HBM roofline per 3D step"
 
@@ -92,8 +109,14 @@ def kernel_times(box, reps: int, precision: str = "exact"):
                                                              precision=precision),
         "advect_transpose13": lambda: ops.advect_transpose13(u2, b2, nx, ny, nz, dt, u, b,
                                                              precision=precision),
+        "transpose12_advect": lambda: ops.transpose12_advect(u2, b2, nx, ny, nz, dt, u, b,
+                                                             precision=precision),
+        "transpose13_advect": lambda: ops.transpose13_advect(u2, b2, nx, ny, nz, dt, u, b,
+                                                             precision=precision),
         "calcfl": lambda: ops.calcfl_device(u, b, nx, ny, nz),
     }
+    if nx % 16:  # the transpose+advectbyzx kernels need 16-pencil blocks
+        del calls["transpose12_advect"], calls["transpose13_advect"]
     out = {}
     with torch.cuda.stream(box.stream):
         for name, f in calls.items():
@@ -148,13 +171,13 @@ def pmc_traffic(op: str, cells: int):
     return None
 
 
-def kernel_table(kt: dict, cells: int, precision: str):
+def kernel_table(kt: dict, cells: int, precision: str, calls: dict):
     """Per kernel (rocprofv3 identity): launches per step, time per step, the
     mean launch, algorithmic bytes per launch and the achieved fraction of
     the HBM peak.  Operators that launch the same kernel are pooled."""
     groups = {}
     for op, t in kt.items():
-        c = CALLS_PER_STEP.get(op, 0)
+        c = calls.get(op, 0)
         if not c:
             continue
         g = groups.setdefault(KERNEL_OF[op], {"ops": [], "launches": 0, "s": 0.0, "bytes": 0.0})
@@ -335,7 +358,7 @@ def main():
     print(f"[bench] {ms_per_step:.2f} ms/step, t={t_sim:.4f} dt={dt:.4g}", file=sys.stderr, flush=True)
 
     kt = kernel_times(box, args.kernel_reps, args.precision)
-    kernels = kernel_table(kt, N, args.precision)
+    kernels = kernel_table(kt, N, args.precision, calls_per_step(n))
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dom]
     step_bytes = sum(ALGO_BYTES[k] * REF_CALLS[k] for k in REF_CALLS) * N

@@ -65,6 +65,10 @@ SIGNATURES = {
                                                C.c_float, C.c_int, _vp]),
     "mhd_advect_transpose13_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
                                                C.c_float, C.c_int, _vp]),
+    "mhd_transpose12_advect_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                               C.c_float, C.c_int, _vp]),
+    "mhd_transpose13_advect_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                               C.c_float, C.c_int, _vp]),
     "mhd_transpose12_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "mhd_transpose13_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "mhd_transpose13_any_async": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),

@@ -21,7 +21,7 @@ LIB = PKG / "libmhd_hip.so"
 SOURCES = ["mhd_fluid.hip", "mhd_advect.hip", "mhd_transpose.hip", "mhd_cfl.hip", "mhd_ic.hip",
            "mhd_capi.hip", "mhd_ensemble.hip", "mhd_corrector.hip",
            "mhd_fused.hip", "mhd_io.hip", "mhd_copy.hip",
-           "mhd_advect_transpose.hip"]
+           "mhd_advect_transpose.hip", "mhd_transpose_advect.hip"]
 
 # -ffp-contract=off: no FMA contraction, so float/double rounding follows
 # the reference's C promotion rules exactly (bitwise parity).

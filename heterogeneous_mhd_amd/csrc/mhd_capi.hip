@@ -326,6 +326,24 @@ int mhd_advect_transpose13_async(float *u, float *b, const float *update_u, cons
                                         precision == MHD_PREC_FAST, 13);
 }
 
+int mhd_transpose12_advect_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream)
+{
+    if (precision != MHD_PREC_EXACT && precision != MHD_PREC_FAST) return MHD_ERR_ARG;
+    return mhd::launch_transpose_advect(u, b, update_u, update_b, nx, ny, nz, dt_dev, dt, (hipStream_t)stream,
+                                        precision == MHD_PREC_FAST, 12);
+}
+
+int mhd_transpose13_advect_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream)
+{
+    if (precision != MHD_PREC_EXACT && precision != MHD_PREC_FAST) return MHD_ERR_ARG;
+    return mhd::launch_transpose_advect(u, b, update_u, update_b, nx, ny, nz, dt_dev, dt, (hipStream_t)stream,
+                                        precision == MHD_PREC_FAST, 13);
+}
+
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream)
 {
@@ -537,14 +555,25 @@ static int enqueue_step(mhd_plan *p, float *U0, float *B0, float *U1, float *B1)
     // :93 advectbyzx and the second half's first pair (:95-97) in one kernel
     RUN(mhd::launch_advect_fluid(U0, B1, U1, B0, nz, nx, ny, dt, 0.0f, s, nullptr, nullptr, nullptr,
                                  0.0, f, 2));                                            // :93-97
-    RUN(mhd::launch_transpose13(U1, B2, U0, B1, nz, nx, ny, s, true));                   // :102
-    // y again, dims (ny, nx, nz)
-    RUN(mhd::launch_advect_fluid(U0, B1, U1, B2, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr,
-                                 0.0, f));                                               // :105-107
-    RUN(mhd::launch_transpose12(U1, B2, U0, B1, ny, nx, nz, s));                         // :112
-    // x again; its sweep also reduces calcfl of the new state for the next step
-    RUN(mhd::launch_advect_fluid(U0, B0, U1, B2, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl,
-                                 f));                                                    // :115-117
+    if (mhd::transpose_advect_supported(U1, U0, B1, nz, nx, ny, 13) &&
+        mhd::transpose_advect_supported(U1, U0, B2, ny, nx, nz, 12)) {
+        // each transpose and the advectbyzx after it in one kernel; the fluid
+        // sweeps stand alone
+        RUN(mhd::launch_transpose_advect(U1, B2, U0, B1, nz, nx, ny, dt, 0.0f, s, f, 13));  // :102, :105
+        RUN(mhd::launch_fluidx(U0, U1, B2, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr, 0.0, f));  // :107
+        RUN(mhd::launch_transpose_advect(U1, B0, U0, B2, ny, nx, nz, dt, 0.0f, s, f, 12));  // :112, :115
+        // x again; its sweep also reduces calcfl of the new state for the next step
+        RUN(mhd::launch_fluidx(U0, U1, B0, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl, f));  // :117
+    } else {
+        RUN(mhd::launch_transpose13(U1, B2, U0, B1, nz, nx, ny, s, true));                   // :102
+        // y again, dims (ny, nx, nz)
+        RUN(mhd::launch_advect_fluid(U0, B1, U1, B2, ny, nx, nz, dt, 0.0f, s, nullptr, nullptr, nullptr,
+                                     0.0, f));                                               // :105-107
+        RUN(mhd::launch_transpose12(U1, B2, U0, B1, ny, nx, nz, s));                         // :112
+        // x again; its sweep also reduces calcfl of the new state for the next step
+        RUN(mhd::launch_advect_fluid(U0, B0, U1, B2, nx, ny, nz, dt, 0.0f, s, p->ws, inv, dtn, p->cfl,
+                                     f));                                                    // :115-117
+    }
 #undef RUN
     return rc;
 }

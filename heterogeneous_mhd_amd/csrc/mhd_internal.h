@@ -117,6 +117,17 @@ int launch_advect_fluid(float *u_out, float *b_out, const float *uu, const float
 // equal to the two launches.  No output may overlap an input.
 int launch_advect_transpose(float *u_out, float *b_out, const float *uu, const float *bu, int n1, int n2, int n3,
                             const float *dt_dev, float dt_val, hipStream_t s, bool fast, int which);
+// transpose12 (which = 12, out dims (n2, n1, n3)) or the general transpose13
+// (which = 13, out dims (n3, n2, n1)) of (uu, bu) followed by advectbyzx of
+// the transposed state (b_main = the transposed b), in one kernel
+// (mhd_transpose_advect.hip); bitwise equal to the two launches.  Needs
+// transpose_advect_supported (n1 a multiple of 16, the output's first dim a
+// multiple of 4, 16-byte aligned u and b); MHD_ERR_SHAPE otherwise.  No
+// output may overlap an input.
+bool transpose_advect_supported(const float *u_out, const float *uu, const float *bu, int n1, int n2, int n3,
+                                int which);
+int launch_transpose_advect(float *u_out, float *b_out, const float *uu, const float *bu, int n1, int n2,
+                            int n3, const float *dt_dev, float dt_val, hipStream_t s, bool fast, int which);
 
 int launch_transpose12(float *u_out, float *b_out, const float *uu, const float *bu, int nx, int ny,
                        int nz, hipStream_t s);

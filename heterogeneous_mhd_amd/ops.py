@@ -117,6 +117,33 @@ def advect_transpose13(u, b, nx, ny, nz, dt, update_u, update_b, precision="exac
                                              _stream()), "advect_transpose13")
 
 
+def transpose12_advect(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
+    """transpose12 then advectbyzx as one kernel (bitwise equal to the two
+    calls, C/mhd.c:112, :115): u, b <- transpose12(update_u), CT update of
+    transpose12(update_b) along its first axis; (nx, ny, nz) are the input
+    dims, the output's are (ny, nx, nz).  Needs nx % 16 == 0 and ny % 4 == 0.
+    No output may overlap an input."""
+    n = nx * ny * nz
+    dp, dv = _dt(dt)
+    check(lib().mhd_transpose12_advect_async(_dev(u, 5 * n, "u"), _dev(b, 3 * n, "b"),
+                                             _dev(update_u, 5 * n, "update_u"),
+                                             _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv,
+                                             MHD_PREC_FAST if precision == "fast" else MHD_PREC_EXACT,
+                                             _stream()), "transpose12_advect")
+
+
+def transpose13_advect(u, b, nx, ny, nz, dt, update_u, update_b, precision="exact"):
+    """The general transpose13 then advectbyzx as one kernel (C/mhd.c:102,
+    :105): out dims (nz, ny, nx).  Needs nx % 16 == 0 and nz % 4 == 0."""
+    n = nx * ny * nz
+    dp, dv = _dt(dt)
+    check(lib().mhd_transpose13_advect_async(_dev(u, 5 * n, "u"), _dev(b, 3 * n, "b"),
+                                             _dev(update_u, 5 * n, "update_u"),
+                                             _dev(update_b, 3 * n, "update_b"), nx, ny, nz, dp, dv,
+                                             MHD_PREC_FAST if precision == "fast" else MHD_PREC_EXACT,
+                                             _stream()), "transpose13_advect")
+
+
 def transpose12(u, b, update_u, update_b, nx, ny, nz, iter=0):
     """x<->y swap of update_* into u, b (C/transpose12.c:7-47)."""
     del iter

@@ -42,6 +42,10 @@ _OPS = [
     (re.compile(r"\badvect_transpose_kernel<\d+, true, 12>"), "advect_transpose12 fast"),
     (re.compile(r"\badvect_transpose_kernel<\d+, false, 13>"), "advect_transpose13"),
     (re.compile(r"\badvect_transpose_kernel<\d+, true, 13>"), "advect_transpose13 fast"),
+    (re.compile(r"\btranspose_advect_kernel<\d+, false, 12>"), "transpose12_advect"),
+    (re.compile(r"\btranspose_advect_kernel<\d+, true, 12>"), "transpose12_advect fast"),
+    (re.compile(r"\btranspose_advect_kernel<\d+, false, 13>"), "transpose13_advect"),
+    (re.compile(r"\btranspose_advect_kernel<\d+, true, 13>"), "transpose13_advect fast"),
     (re.compile(r"\bstream_copy_kernel"), "copy_matrix (streaming copy)"),
     (re.compile(r"\btranspose_kernel(?!<)"), "transpose (u and b)"),
     (re.compile(r"\btranspose_kernel<5>"), "transpose (u, 5 comps)"),
@@ -66,6 +70,7 @@ _OPS = [
     (re.compile(r"\bconv_dw_kernel"), "corr_conv_dw"),
     (re.compile(r"\bdw_reduce_kernel"), "corr_dw_reduce"),
     (re.compile(r"\bpack_kernel"), "corr_pack"),
+    (re.compile(r"\bpack_features_kernel"), "corr_pack_features"),
     (re.compile(r"\b(ysum|ymean|apply|check|decide|commit)_kernel"), "corr_apply"),
     (re.compile(r"\bweight_prep_kernel"), "corr_weight_prep"),
     (re.compile(r"\bic_pass\d"), "ic"),

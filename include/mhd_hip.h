@@ -142,6 +142,21 @@ int mhd_advect_transpose12_async(float *u, float *b, const float *update_u, cons
 int mhd_advect_transpose13_async(float *u, float *b, const float *update_u, const float *update_b,
                                  int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
                                  mhd_stream_t stream);
+/* transpose12 of (update_u, update_b) followed by advectbyzx of the
+ * transposed state along its first axis (b_main = the transposed b), in one
+ * kernel, bitwise equal to the two calls -- C/mhd.c:112 and :115 (no
+ * reference counterpart as one call).  u receives the transposed u and b
+ * the new field, dims (ny, nx, nz).  Needs nx a multiple of 16, ny a
+ * multiple of 4 and 16-byte aligned pointers (MHD_ERR_SHAPE otherwise).  No
+ * output may overlap an input.  precision: MHD_PREC_EXACT or MHD_PREC_FAST. */
+int mhd_transpose12_advect_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream);
+/* The same with the general transpose13 (out dims (nz, ny, nx)) --
+ * C/mhd.c:102 and :105.  Needs nx a multiple of 16 and nz of 4. */
+int mhd_transpose13_advect_async(float *u, float *b, const float *update_u, const float *update_b,
+                                 int nx, int ny, int nz, const float *dt_dev, float dt, int precision,
+                                 mhd_stream_t stream);
 int mhd_transpose12_async(float *u, float *b, const float *update_u, const float *update_b, int nx,
                           int ny, int nz, mhd_stream_t stream);
 int mhd_transpose13_async(float *u, float *b, const float *update_u, const float *update_b, int nx,

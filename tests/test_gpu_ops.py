@@ -315,6 +315,70 @@ def test_advect_transpose_equals_the_two_calls(gpu, shape, which, precision):
         assert bits_equal(_h(fu), ru) and bits_equal(_h(fb), rb)
 
 
+# input dims; the fused form needs nx % 16 == 0 and the output's first dim (ny
+# for transpose12, nz for transpose13) a multiple of 4: full and partial
+# tiles along the sweep axis, sweep axes shorter than the 64-cell window (4:
+# many wraps), partner extents of one and several pencil blocks, a remaining
+# axis of extent 1 (the look-ahead row wraps twice), non-cubic boxes
+TA_SHAPES = [(16, 16, 16), (16, 24, 16), (32, 8, 4), (16, 4, 8), (48, 64, 20), (16, 8, 1), (16, 1, 8),
+             (16, 100, 12), (64, 56, 56), (32, 4, 4), (16, 12, 100), (1024, 8, 4), (16, 20, 16)]
+TA_CASES = [(s, w) for s in TA_SHAPES for w in (12, 13) if (s[1] if w == 12 else s[2]) % 4 == 0]
+
+
+@pytest.mark.parametrize("shape,which", TA_CASES, ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple)
+                         else str(v))
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_transpose_advect_equals_the_two_calls(gpu, shape, which, precision):
+    """transpose12_advect / transpose13_advect is bitwise transpose12 (resp.
+    the general transpose13) followed by advectbyzx of the transposed state:
+    C/mhd.c:112, :115 and :102, :105; in exact mode also the oracle's two
+    calls."""
+    import torch
+    from heterogeneous_mhd_amd import ops
+    from oracle import Oracle
+    nx, ny, nz = shape
+    od = (ny, nx, nz) if which == 12 else (nz, ny, nx)
+    print(f"transpose{which}_advect {nx}x{ny}x{nz} {precision}", flush=True)
+    u, b = _random_state(nx, ny, nz, seed=2 * nx + ny + 5 * nz)
+    dt = np.float32(0.05)
+    ud, bd = _t(u, gpu), _t(b, gpu)
+    tu, tb = torch.empty_like(ud), torch.empty_like(bd)
+    if which == 12:
+        ops.transpose12(tu, tb, ud, bd, nx, ny, nz)
+    else:
+        ops.transpose13_any(tu, tb, ud, bd, nx, ny, nz)
+    b1 = tb.clone()
+    ops.advectbyzx(tu, b1, *od, float(dt), tu, tb, precision=precision)
+    fu, fb = torch.full_like(ud, np.nan), torch.full_like(bd, np.nan)
+    getattr(ops, f"transpose{which}_advect")(fu, fb, nx, ny, nz, float(dt), ud, bd, precision=precision)
+    assert bits_equal(_h(fu), _h(tu))
+    assert bits_equal(_h(fb), _h(b1))
+    if precision == "exact":
+        O = Oracle("f32")
+        ru, rb = O.transpose12(u, b, nx, ny, nz) if which == 12 else O.transpose13_any(u, b, nx, ny, nz)
+        assert bits_equal(_h(fu), ru)
+        assert bits_equal(_h(fb), O.advectbyzx(ru, rb, *od, dt))
+
+
+def test_transpose_advect_rejects_unsupported_shapes_and_overlap(gpu):
+    import torch
+    from heterogeneous_mhd_amd import MHDError, ops
+    for shape, which in [((8, 8, 8), 12), ((24, 16, 16), 13), ((16, 6, 16), 12), ((16, 16, 6), 13)]:
+        nx, ny, nz = shape
+        u, b = _random_state(nx, ny, nz, seed=3)
+        ud, bd = _t(u, gpu), _t(b, gpu)
+        with pytest.raises(MHDError):
+            getattr(ops, f"transpose{which}_advect")(torch.empty_like(ud), torch.empty_like(bd), nx, ny, nz,
+                                                     0.05, ud, bd)
+    n = 16
+    u, b = _random_state(n, n, n, seed=1)
+    ud, bd = _t(u, gpu), _t(b, gpu)
+    with pytest.raises(MHDError):
+        ops.transpose12_advect(ud, torch.empty_like(bd), n, n, n, 0.05, ud, bd)
+    with pytest.raises(MHDError):
+        ops.transpose13_advect(torch.empty_like(ud), bd, n, n, n, 0.05, ud, bd)
+
+
 def test_advect_transpose_rejects_overlap(gpu):
     import torch
     from heterogeneous_mhd_amd import MHDError, ops

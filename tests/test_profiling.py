@@ -18,6 +18,10 @@ def test_single_box_and_ensemble_kernels_are_told_apart():
         ANON + "advect_fluid_kernel<512, 2, true, true, 512, true, 1>(float*)":
             "advect_fluidx+calcfl fast",
         ANON + "advect_fluid_kernel<512, 2, false, true, 512, true, 2>(float*)": "advect2_fluidx fast",
+        ANON + "advect_transpose_kernel<512, true, 13>(float*)": "advect_transpose13 fast",
+        ANON + "transpose_advect_kernel<512, true, 12>(float*)": "transpose12_advect fast",
+        ANON + "transpose_advect_kernel<512, false, 13>(float*)": "transpose13_advect",
+        ANON + "fluidx_kernel<512, 2, true, true, 512, true>(float*)": "fluidx+calcfl fast",
     }
     for name, op in cases.items():
         assert op_of(name) == op, (name, op_of(name), op)
