@@ -20,11 +20,13 @@ namespace {
 
 // The two-pass kernel is held to >= 6 waves per SIMD (<= 80 VGPRs, no
 // spills): three 512-thread workgroups per CU instead of the two its 86
-// VGPRs would allow, measured 30.5 -> 24.1 ms per 1024^3 launch.  The
-// one-pass kernels keep the compiler's choice (the bound makes their CFL
-// variants spill, and 64 VGPRs for a fourth workgroup gave no gain).
+// VGPRs would allow, measured 30.5 -> 24.1 ms per 1024^3 launch.  So is the
+// FP32 one-pass kernel with the fused calcfl (81 VGPRs unbounded).  The
+// other one-pass kernels keep the compiler's choice (64 VGPRs for a fourth
+// workgroup gave no gain).
 template <int BLOCK, int M, bool CFL, bool FAST, int TP, bool FIT, int NPASS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(NPASS == 2 ? 6 : 1)))
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(NPASS == 2 || (CFL && FAST && BLOCK <= 512) ? 6 : 1)))
 void advect_fluid_kernel(float *u_out, float *b_out,
                                                              const float *__restrict__ uu,
                                                              const float *__restrict__ bu, int nx,
