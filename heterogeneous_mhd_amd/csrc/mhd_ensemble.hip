@@ -22,7 +22,9 @@
 // state is saved before and restored after, so tuning does not move it.
 #include "../../include/mhd_hip.h"
 #include "mhd_advect_body.h"
+#include "mhd_advect_transpose_body.h"
 #include "mhd_fluid_body.h"
+#include "mhd_transpose_advect_body.h"
 #include "mhd_transpose_body.h"
 
 #include <string.h>
@@ -41,6 +43,7 @@ struct EnsDev {
     const long long *boff;   // float offset of each box in the b buffers
     const int *gpre;         // prefix sums of sweep pencil groups per box (nbox+1)
     const int *tpre;         // prefix sums of transpose tiles per box (nbox+1)
+    const int *lpre;         // prefix sums of advectbyzx+transpose tiles per box (nbox+1)
     float *t, *dt, *dtn, *inv;
 };
 
@@ -142,6 +145,61 @@ __global__ __launch_bounds__(kTB) void ens_transpose_kernel(EnsDev E, float *out
             tile_fast<NC>(o, s, tile, i0, a0, zz, sa_in, sz_in, si_out, sz_out, perm);
         else
             tile_scalar<NC>(o, s, tile, i0, a0, ni, na, zz, sa_in, sz_in, si_out, sz_out, perm);
+    }
+}
+
+// The tiles of the fused advectbyzx+transpose (first half) and
+// transpose+advectbyzx (second half) kernels, for boxes whose side is a
+// multiple of kTP: 56 cells along the sweep axis x 16 pencils x a run of at
+// most 64 steps of the remaining axis, as the single-box launchers cut them,
+// so every box evolves bitwise as it would alone.
+__host__ __device__ inline int ens_tile_qlen(int n) { return n < 64 ? n : 64; }
+__host__ __device__ inline int ens_tiles(int n)
+{
+    return (n + at_tile::kTI - 1) / at_tile::kTI * (n / at_tile::kTP) * ((n + ens_tile_qlen(n) - 1) / ens_tile_qlen(n));
+}
+
+template <bool FAST, int WHICH>
+__global__ __launch_bounds__(512) void ens_advect_transpose_kernel(EnsDev E, float *Uo, float *Bo, const float *Ui,
+                                                                   const float *Bi)
+{
+    const int total = E.lpre[E.nbox];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int box = find_box(E.lpre, E.nbox, item);
+        const int n = E.n[box];
+        at_tile::AtGeom g;
+        g.n1 = g.n2 = g.n3 = g.np = g.nq = n;
+        g.nti = (n + at_tile::kTI - 1) / at_tile::kTI;
+        g.npb = n / at_tile::kTP;
+        g.qlen = ens_tile_qlen(n);
+        g.nqc = (n + g.qlen - 1) / g.qlen;
+        g.nwg = 0;
+        const long long uo = E.uoff[box], bo = E.boff[box];
+        __syncthreads();  // the previous item's last LDS reads are done
+        at_tile::advect_transpose_tile<512, FAST, WHICH>(Uo + uo, Bo + bo, Ui + uo, Bi + bo, g,
+                                                         (unsigned)(item - E.lpre[box]), E.dt[box]);
+    }
+}
+
+template <bool FAST, int WHICH>
+__global__ __launch_bounds__(512) void ens_transpose_advect_kernel(EnsDev E, float *Uo, float *Bo, const float *Ui,
+                                                                   const float *Bi)
+{
+    const int total = E.lpre[E.nbox];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int box = find_box(E.lpre, E.nbox, item);
+        const int n = E.n[box];
+        ta_tile::TaGeom g;
+        g.ns = g.np = g.nq = n;
+        g.nti = (n + ta_tile::kTI - 1) / ta_tile::kTI;
+        g.npb = n / ta_tile::kTP;
+        g.qlen = ens_tile_qlen(n);
+        g.nqc = (n + g.qlen - 1) / g.qlen;
+        g.nwg = 0;
+        const long long uo = E.uoff[box], bo = E.boff[box];
+        __syncthreads();  // the previous item's last LDS reads are done
+        ta_tile::transpose_advect_tile<512, FAST, WHICH>(Uo + uo, Bo + bo, Ui + uo, Bi + bo, g,
+                                                         (unsigned)(item - E.lpre[box]), E.dt[box]);
     }
 }
 
@@ -336,6 +394,27 @@ int launch_transpose(const EnsDev &E, float *Uo, float *Bo, const float *Ui, con
     return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
 }
 
+// the fused tile kernels: first = true for advectbyzx+transpose (first
+// half), false for transpose+advectbyzx (second half); grid: persistent
+// workgroups (two fit per CU)
+int launch_tiles(const EnsDev &E, bool first, bool fast, int which, float *Uo, float *Bo, const float *Ui,
+                 const float *Bi, int grid, hipStream_t s)
+{
+    (void)hipGetLastError();
+    auto kernel = [&]() -> const void * {
+        if (first && which == 12)
+            return fast ? (const void *)ens_advect_transpose_kernel<true, 12> : (const void *)ens_advect_transpose_kernel<false, 12>;
+        if (first)
+            return fast ? (const void *)ens_advect_transpose_kernel<true, 13> : (const void *)ens_advect_transpose_kernel<false, 13>;
+        if (which == 12)
+            return fast ? (const void *)ens_transpose_advect_kernel<true, 12> : (const void *)ens_transpose_advect_kernel<false, 12>;
+        return fast ? (const void *)ens_transpose_advect_kernel<true, 13> : (const void *)ens_transpose_advect_kernel<false, 13>;
+    };
+    void *args[] = {(void *)&E, (void *)&Uo, (void *)&Bo, (void *)&Ui, (void *)&Bi};
+    if (hipLaunchKernel(kernel(), dim3(grid), dim3(512), args, 0, s) != hipSuccess) return MHD_ERR_HIP;
+    return hipGetLastError() == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
 }  // namespace
 }  // namespace mhd
 
@@ -353,6 +432,8 @@ struct mhd_ensemble {
     int shape, grid;                    // current launch shape and persistent grid size
     int split, nsmall, nfit, nfit1, total;  // see sweep_split (set with the shape)
     bool fast;                          // FP32 tolerance mode of the sweeps
+    bool tiles;                         // every side a multiple of 16: the fused tile schedule
+    int tile_items;                     // work items of one tile launch
     // device tables
     void *tab;
     EnsDev E;
@@ -392,6 +473,28 @@ int ens_step(mhd_ensemble *e)
     (void)hipGetLastError();
     hipLaunchKernelGGL(ens_begin_kernel, dim3((e->nbox + 63) / 64), dim3(64), 0, s, E, e->tf);  // :63-64
     RUN(ens_error(hipGetLastError()));
+    if (e->tiles) {
+        // the order of C/mhd.c:71-117 with every transpose fused with the
+        // advectbyzx beside it, as the single-box plan runs it; the step ends
+        // in (U0, B0)
+        const int tg = e->tile_items < 2 * e->ncu ? e->tile_items : 2 * e->ncu;
+        RUN(sweep(sh, 0, false, f, E, U1, nullptr, U0, B0, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));  // :71
+        RUN(launch_tiles(E, true, f, 12, U0, B1, U1, B0, tg, s));                                          // :74, :78
+        RUN(sweep(sh, 0, false, f, E, U0, nullptr, U0, B1, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));  // :80
+        RUN(launch_tiles(E, true, f, 13, U1, B0, U0, B1, tg, s));                                          // :84, :88
+        RUN(sweep(sh, 0, false, f, E, U1, nullptr, U1, B0, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));  // :91
+        RUN(sweep(sh, 1, false, f, E, nullptr, B1, U1, B0, B0, nm, ns, nf, sp, nf1, tot, nullptr, g, s));       // :93
+        RUN(sweep(sh, 2, false, f, E, U0, B0, U1, B1, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));       // :95-97
+        RUN(launch_tiles(E, false, f, 13, U1, B1, U0, B0, tg, s));                                         // :102, :105
+        RUN(sweep(sh, 0, false, f, E, U1, nullptr, U1, B1, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));  // :107
+        RUN(launch_tiles(E, false, f, 12, U0, B0, U1, B1, tg, s));                                         // :112, :115
+        RUN(sweep(sh, 0, true, f, E, U0, nullptr, U0, B0, nullptr, nm, ns, nf, sp, nf1, tot, e->ws, g, s));    // :117 + next calcfl
+        if (rc == MHD_OK) {
+            hipLaunchKernelGGL(ens_cfl_fold_kernel, dim3(e->nbox), dim3(kFoldBlock), 0, s, E, e->ws, e->cfl);
+            rc = ens_error(hipGetLastError());
+        }
+        return rc;
+    }
     // the order of C/mhd.c:71-117; the second half's advectbyzx + fluidx pairs
     // are fused, and u ping-pongs so that the step ends in (U0, B0)
     RUN(sweep(sh, 0, false, f, E, U1, nullptr, U0, B0, nullptr, nm, ns, nf, sp, nf1, tot, nullptr, g, s));   // :71
@@ -485,7 +588,7 @@ int mhd_ensemble_create(mhd_ensemble **out, int nbox, const int *sides, double c
         }
     e->grid = 4 * e->ncu;
     // device tables, in pack order for the work lists
-    const size_t tb = sizeof(int) * nbox + 2 * sizeof(long long) * nbox + 2 * sizeof(int) * (nbox + 1) +
+    const size_t tb = sizeof(int) * nbox + 2 * sizeof(long long) * nbox + 3 * sizeof(int) * (nbox + 1) +
                       4 * sizeof(float) * nbox + 256;
     if (hipMalloc(&e->tab, tb) != hipSuccess) {
         delete e;
@@ -497,9 +600,11 @@ int mhd_ensemble_create(mhd_ensemble **out, int nbox, const int *sides, double c
     int *d_n = (int *)(d_boff + nbox);
     int *d_gpre = d_n + nbox;
     int *d_tpre = d_gpre + nbox + 1;
-    float *d_f = (float *)(d_tpre + nbox + 1);
+    int *d_lpre = d_tpre + nbox + 1;
+    float *d_f = (float *)(d_lpre + nbox + 1);
     std::vector<long long> hu(nbox), hb(nbox);
-    std::vector<int> hn(nbox), ht(nbox + 1, 0);
+    std::vector<int> hn(nbox), ht(nbox + 1, 0), hl(nbox + 1, 0);
+    e->tiles = true;
     for (int q = 0; q < nbox; q++) {
         const int b = e->order[q];
         hu[q] = e->uoff[b];
@@ -507,11 +612,15 @@ int mhd_ensemble_create(mhd_ensemble **out, int nbox, const int *sides, double c
         hn[q] = e->n[b];
         const int nt = (hn[q] + 31) / 32;
         ht[q + 1] = ht[q] + nt * nt * hn[q];
+        if (hn[q] % at_tile::kTP) e->tiles = false;
+        hl[q + 1] = hl[q] + (hn[q] % at_tile::kTP ? 0 : ens_tiles(hn[q]));
     }
     hipError_t er = hipMemcpy(d_uoff, hu.data(), sizeof(long long) * nbox, hipMemcpyHostToDevice);
     if (er == hipSuccess) er = hipMemcpy(d_boff, hb.data(), sizeof(long long) * nbox, hipMemcpyHostToDevice);
     if (er == hipSuccess) er = hipMemcpy(d_n, hn.data(), sizeof(int) * nbox, hipMemcpyHostToDevice);
     if (er == hipSuccess) er = hipMemcpy(d_tpre, ht.data(), sizeof(int) * (nbox + 1), hipMemcpyHostToDevice);
+    if (er == hipSuccess) er = hipMemcpy(d_lpre, hl.data(), sizeof(int) * (nbox + 1), hipMemcpyHostToDevice);
+    e->tile_items = hl[nbox];
     if (er == hipSuccess) er = hipMemset(d_f, 0, 4 * sizeof(float) * nbox);
     e->E.nbox = nbox;
     e->E.n = d_n;
@@ -519,6 +628,7 @@ int mhd_ensemble_create(mhd_ensemble **out, int nbox, const int *sides, double c
     e->E.boff = d_boff;
     e->E.gpre = d_gpre;
     e->E.tpre = d_tpre;
+    e->E.lpre = d_lpre;
     e->E.t = d_f;
     e->E.dt = d_f + nbox;
     e->E.dtn = d_f + 2 * nbox;
