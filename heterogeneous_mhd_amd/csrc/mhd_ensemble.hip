@@ -151,8 +151,9 @@ __global__ __launch_bounds__(kTB) void ens_transpose_kernel(EnsDev E, float *out
 // The tiles of the fused advectbyzx+transpose (first half) and
 // transpose+advectbyzx (second half) kernels, for boxes whose side is a
 // multiple of kTP: 56 cells along the sweep axis x 16 pencils x a run of at
-// most 64 steps of the remaining axis, as the single-box launchers cut them,
-// so every box evolves bitwise as it would alone.
+// most 64 steps of the remaining axis.  Each run starts its walk from the
+// inputs alone, so how a box is cut into runs does not change a bit of its
+// result: every box evolves as it would alone.
 __host__ __device__ inline int ens_tile_qlen(int n) { return n < 64 ? n : 64; }
 __host__ __device__ inline int ens_tiles(int n)
 {
