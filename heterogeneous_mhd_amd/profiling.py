@@ -59,6 +59,10 @@ _OPS = [
     (re.compile(r"ens_advect_kernel"), "ensemble advectbyzx"),
     (re.compile(r"ens_fused_kernel<\d+, \d+, \d+, true"), "ensemble advect_fluidx+calcfl"),
     (re.compile(r"ens_fused_kernel"), "ensemble advect_fluidx"),
+    (re.compile(r"ens_advect_transpose_kernel<\w+, 12>"), "ensemble advect_transpose12"),
+    (re.compile(r"ens_advect_transpose_kernel<\w+, 13>"), "ensemble advect_transpose13"),
+    (re.compile(r"ens_transpose_advect_kernel<\w+, 12>"), "ensemble transpose12_advect"),
+    (re.compile(r"ens_transpose_advect_kernel<\w+, 13>"), "ensemble transpose13_advect"),
     (re.compile(r"ens_transpose_kernel<5>"), "ensemble transpose (u)"),
     (re.compile(r"ens_transpose_kernel<3>"), "ensemble transpose (b)"),
     (re.compile(r"ens_cfl_fold_kernel"), "ensemble calcfl fold"),

@@ -22,6 +22,10 @@ def test_single_box_and_ensemble_kernels_are_told_apart():
         ANON + "transpose_advect_kernel<512, true, 12>(float*)": "transpose12_advect fast",
         ANON + "transpose_advect_kernel<512, false, 13>(float*)": "transpose13_advect",
         ANON + "fluidx_kernel<512, 2, true, true, 512, true>(float*)": "fluidx+calcfl fast",
+        ANON + "ens_advect_transpose_kernel<true, 13>(mhd::(anonymous namespace)::EnsDev)":
+            "ensemble advect_transpose13",
+        ANON + "ens_transpose_advect_kernel<false, 12>(mhd::(anonymous namespace)::EnsDev)":
+            "ensemble transpose12_advect",
     }
     for name, op in cases.items():
         assert op_of(name) == op, (name, op_of(name), op)
