@@ -156,8 +156,10 @@ __device__ __forceinline__ unsigned dpp_u32(unsigned v)
 // integers and the max is exact in any order: a DPP butterfly within each
 // row of 16 lanes (quad swaps, half-row and row mirrors), two lane swaps
 // across rows, and, when a pencil spans several wavefronts, one LDS
-// exchange.  Ends with a workgroup barrier, which the caller relies on to
-// recycle the LDS stencil arrays.
+// exchange.  Returns after a workgroup barrier that follows every LDS read
+// this thread made before the call, so the caller may then overwrite the
+// stencil arrays: the exchange's own barrier when a pencil spans several
+// wavefronts, else one at the end.
 template <int BLOCK>
 __device__ __forceinline__ float pencil_max(float xf, int tp, lds_f *red)
 {
@@ -176,8 +178,9 @@ __device__ __forceinline__ float pencil_max(float xf, int tp, lds_f *red)
         const int wpp = tp >> 6, first = (threadIdx.x / tp) * wpp;
         x = r[first];
         for (int w = 1; w < wpp; w++) x = max(x, r[first + w]);
+    } else {
+        __syncthreads();
     }
-    __syncthreads();
     return __uint_as_float(x);
 }
 
@@ -315,7 +318,11 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
     // ---- stage 1: half-step predictor, C/tvd1.c:36-49
     // s[][] holds, in turn, fr of stage 1, the predictor u1, fr of stage 2 and
     // the final flux; q keeps the sweep's input u, the base of the update (:116).
-    float s[M][5];
+    // Each exchange writes both wave halves, and each thread then reads every
+    // neighbour value it needs in one round after one barrier, evaluating the
+    // stencil terms of its neighbours (flux(i-1); dfrm, dflm at i+1) itself,
+    // with the same expressions, instead of a further LDS round trip.
+    float s[M][5], t[M][5];
     {
         const bool norm = c1 > 0.0f;
         const bool fast = c1 >= 0x1p-30f && c1 <= 0x1p30f;
@@ -332,28 +339,18 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
                         // fluxes, which the pencil-uniform factor g gives
                         const float vg = v[r][c] * g;
                         s[r][c] = fmaf(c1, q[r][c], vg);
-                        Ap[5 * i + c] = fmaf(c1, q[r][c], -vg);
+                        t[r][c] = fmaf(c1, q[r][c], -vg);
                     } else {
                         const float vv = norm ? div_c(v[r][c], c1, rc, fast) : v[r][c];
                         const float wr = q[r][c] + vv + z;
                         const float wl = q[r][c] - vv + z;
                         s[r][c] = scale0(c1, wr);          // fr
-                        Ap[5 * i + c] = scale0(c1, wl);    // fl(i-1) = c*wl(i)
+                        t[r][c] = scale0(c1, wl);          // fl(i-1) = c*wl(i)
                     }
+                    Ap[5 * i + c] = t[r][c];
+                    Bp[5 * i + c] = s[r][c];
                 }
             }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1;
-#pragma unroll
-            for (int c = 0; c < 5; c++)  // flux
-                Bp[5 * i + c] = FAST ? s[r][c] - Ap[5 * ip + c]  // twice the flux, see hdt
-                                     : h * s[r][c] + (-h) * Ap[5 * ip + c] + z;
         }
     }
     __syncthreads();
@@ -369,13 +366,18 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
         if (live && i < nx) {
-            const int im = i == 0 ? nx - 1 : i - 1;
+            const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
 #pragma unroll
             for (int c = 0; c < 5; c++) {
+                // flux(i) = (fr(i) - fl(i+1)) / 2 and flux(i-1)
+                const float fl1 = Ap[5 * ip + c], frm = Bp[5 * im + c];
                 if constexpr (FAST) {
-                    s[r][c] = fmaf(hdt2, Bp[5 * i + c] - Bp[5 * im + c], q[r][c]);
+                    const float f0 = s[r][c] - fl1, fm = frm - t[r][c];  // twice the fluxes, see hdt
+                    s[r][c] = fmaf(hdt2, f0 - fm, q[r][c]);
                 } else {
-                    const float d = Bp[5 * i + c] - Bp[5 * im + c] + z;
+                    const float f0 = h * s[r][c] + (-h) * fl1 + z;
+                    const float fm = h * frm + (-h) * t[r][c] + z;
+                    const float d = f0 - fm + z;
                     s[r][c] = q[r][c] + hdt * d + z;           // u1
                 }
             }
@@ -400,73 +402,47 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
                     if constexpr (FAST) {
                         const float u1 = s[r][c], vg = v[r][c] * g;
                         s[r][c] = fmaf(c2, u1, vg);
-                        Bp[5 * i + c] = fmaf(c2, u1, -vg);
-                        Ap[5 * i + c] = s[r][c];
+                        t[r][c] = fmaf(c2, u1, -vg);
                     } else {
                         const float vv = norm ? div_c(v[r][c], c2, rc, fast) : v[r][c];
                         const float wr = s[r][c] + vv + z;
                         const float wl = s[r][c] - vv + z;
                         s[r][c] = scale0(c2, wr);          // fr
-                        Ap[5 * i + c] = s[r][c];
-                        Bp[5 * i + c] = scale0(c2, wl);    // fl(i-1)
+                        t[r][c] = scale0(c2, wl);          // fl(i-1)
                     }
+                    Ap[5 * i + c] = s[r][c];
+                    Bp[5 * i + c] = t[r][c];
                 }
             }
         }
     }
     __syncthreads();
     // D(i) = dfrm(i) = dfrp(i-1); E(i) = dflm(i) = dflp(i-1); t1 = fr - fl
-    float D[M][5], E[M][5];
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const int i = lt + r * tp;
         if (live && i < nx) {
             const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+            const int ip2 = ip + 1 == nx ? 0 : ip + 1;
 #pragma unroll
             for (int c = 0; c < 5; c++) {
-                const float fr = s[r][c], frm = Ap[5 * im + c];
-                const float flm = Bp[5 * i + c], fl0 = Bp[5 * ip + c];
+                const float fr = s[r][c], fl = t[r][c];
+                const float frm = Ap[5 * im + c], frp = Ap[5 * ip + c];
+                const float flp = Bp[5 * ip + c], flp2 = Bp[5 * ip2 + c];
                 if constexpr (FAST) {
-                    D[r][c] = fr - frm;   // twice dfrm
-                    E[r][c] = flm - fl0;  // twice dflm
-                    s[r][c] = fr - fl0;
+                    const float D = fr - frm, Dp = frp - fr;    // twice dfrm at i, i+1
+                    const float E = fl - flp, Ep = flp - flp2;  // twice dflm at i, i+1
+                    const float dfr = limiter_fast(Dp, D);
+                    const float dfl = limiter_fast(Ep, E);
+                    s[r][c] = fmaf(0.5f, dfr - dfl, fr - flp);  // twice the flux
                 } else {
-                    D[r][c] = h * fr + (-h) * frm + z;
-                    E[r][c] = h * flm + (-h) * fl0 + z;
-                    s[r][c] = fr - fl0 + z;                // t1
-                }
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                Ap[5 * i + c] = D[r][c];
-                Bp[5 * i + c] = E[r][c];
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < M; r++) {
-        const int i = lt + r * tp;
-        if (live && i < nx) {
-            const int ip = i + 1 == nx ? 0 : i + 1;
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                if constexpr (FAST) {
-                    const float dfr = limiter_fast(Ap[5 * ip + c], D[r][c]);
-                    const float dfl = limiter_fast(Bp[5 * ip + c], E[r][c]);
-                    s[r][c] = fmaf(0.5f, dfr - dfl, s[r][c]);  // twice the flux
-                } else {
-                    const float dfr = limiter(Ap[5 * ip + c], D[r][c]);   // (dfrp, dfrm)
-                    const float dfl = limiter(Bp[5 * ip + c], E[r][c]);   // (dflp, dflm)
+                    const float D = h * fr + (-h) * frm + z, Dp = h * frp + (-h) * fr + z;
+                    const float E = h * fl + (-h) * flp + z, Ep = h * flp + (-h) * flp2 + z;
+                    const float t1 = fr - flp + z;
+                    const float dfr = limiter(Dp, D);   // (dfrp, dfrm)
+                    const float dfl = limiter(Ep, E);   // (dflp, dflm)
                     const float t2 = dfr - dfl + z;
-                    s[r][c] = h * s[r][c] + h * t2 + z;                   // flux
+                    s[r][c] = h * t1 + h * t2 + z;      // flux
                 }
             }
         }
@@ -497,9 +473,9 @@ __device__ __forceinline__ void fluid_group(lds_f *lds, lds_f *red, float *u_out
 #pragma unroll
             for (int c = 0; c < 5; c++) {
                 if constexpr (FAST) {
-                    un[c] = fmaf(fdt2, Ap[5 * i + c] - Ap[5 * im + c], q[r][c]);
+                    un[c] = fmaf(fdt2, s[r][c] - Ap[5 * im + c], q[r][c]);
                 } else {
-                    const float d = Ap[5 * i + c] - Ap[5 * im + c] + z;
+                    const float d = s[r][c] - Ap[5 * im + c] + z;
                     un[c] = q[r][c] + fdt * d + z;
                 }
                 Bp[5 * i + c] = un[c];
