@@ -84,6 +84,25 @@ __device__ __forceinline__ int at_perm_u(int c)
     return c == 1 ? 3 : (c == 3 ? 1 : c);
 }
 
+// Store of one value of an output row segment [seg0, seg1) (bytes).  A
+// segment's first or last 128-byte line is shared with the segment of the
+// neighbouring pencil block, which another workgroup writes at about the
+// same time.  Those lines get ordinary stores, which the L2 merges before it
+// writes the line back; with nontemporal stores of both halves the tile
+// wrote 38-42 instead of 32 bytes per cell (FP32, 1024^3).  The lines the
+// segment covers alone get nontemporal stores, which keep the L2 for the
+// walk's reads.
+template <class T>
+__device__ __forceinline__ void st_seg(float *base, unsigned off, T v, uintptr_t seg0, uintptr_t seg1)
+{
+    T *const p = reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+    const uintptr_t line = reinterpret_cast<uintptr_t>(p) & ~(uintptr_t)127;
+    if (line < seg0 || line + 128 > seg1)
+        *p = v;
+    else
+        __builtin_nontemporal_store(v, p);
+}
+
 // Everything one step of the walk reads of the pencils at one q (row q of
 // the tile's pencil block), issued two steps before its output: rho, m1 and
 // both transverse fields over the window for the lines, the halo pencils
@@ -300,21 +319,23 @@ __device__ __forceinline__ void advect_transpose_tile(float *__restrict__ u_out,
                 const int idx = threadIdx.x + r * BLOCK;
                 const int p = idx % kTP, il = idx / kTP;
                 if (ok_b[r]) {
+                    const uintptr_t seg0 = reinterpret_cast<uintptr_t>(ob_base) + (uintptr_t)il * ostride * 12;
+                    const uintptr_t seg1 = seg0 + (uintptr_t)ntp * 12;
                     const float eS0 = eS[p * kPad + il], eS1 = eS[(p + 1) * kPad + il];
                     const float eo = eO[p * kPad + il], en = eN[p * kPad + il];
                     const float fs = fS[p * kPad + il], fo = fO[p * kPad + il];
                     if constexpr (WHICH == 12) {
                         // b1: A1, A2 (the S lines), then B1, B2 (O, N); out (b2, b1, b3)
                         const float b1n = (((b1q[r] - eS0) + eS1) - eo) + en;
-                        st_at<float>(ob_base, off_ob[r], fs);
-                        st_at<float>(ob_base, off_ob[r] + 4u, b1n);
-                        st_at<float>(ob_base, off_ob[r] + 8u, fo);
+                        st_seg<float>(ob_base, off_ob[r], fs, seg0, seg1);
+                        st_seg<float>(ob_base, off_ob[r] + 4u, b1n, seg0, seg1);
+                        st_seg<float>(ob_base, off_ob[r] + 8u, fo, seg0, seg1);
                     } else {
                         // b1: A1, A2 (O, N), then B1, B2 (the S lines); out (b3, b2, b1)
                         const float b1n = (((b1q[r] - eo) + en) - eS0) + eS1;
-                        st_at<float>(ob_base, off_ob[r], fs);
-                        st_at<float>(ob_base, off_ob[r] + 4u, fo);
-                        st_at<float>(ob_base, off_ob[r] + 8u, b1n);
+                        st_seg<float>(ob_base, off_ob[r], fs, seg0, seg1);
+                        st_seg<float>(ob_base, off_ob[r] + 4u, fo, seg0, seg1);
+                        st_seg<float>(ob_base, off_ob[r] + 8u, b1n, seg0, seg1);
                     }
                 }
             }
@@ -335,7 +356,9 @@ __device__ __forceinline__ void advect_transpose_tile(float *__restrict__ u_out,
                     wv.y = src(il, 4 * v + 1);
                     wv.z = src(il, 4 * v + 2);
                     wv.w = src(il, 4 * v + 3);
-                    st_at<v4f>(ou_base, (unsigned)((size_t)il * ostride * 20) + 16u * (unsigned)v, wv);
+                    const uintptr_t seg0 = reinterpret_cast<uintptr_t>(ou_base) + (uintptr_t)il * ostride * 20;
+                    st_seg<v4f>(ou_base, (unsigned)((size_t)il * ostride * 20) + 16u * (unsigned)v, wv, seg0,
+                                seg0 + kTP * 20);
                 }
             } else {
                 for (int f = threadIdx.x; f < nti * ntp * 5; f += BLOCK) {
