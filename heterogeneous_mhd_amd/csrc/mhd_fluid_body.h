@@ -25,8 +25,10 @@
 //    R and NaN for infinite or NaN R.
 //  * dfrp(i) and dfrm(i+1) (C/tvd1.c:60-64) are the same rounded expression
 //    of fr(i+1), fr(i); likewise dflm(i+1) and dflp(i) (:86-90); and
-//    fl(i) = c*wl(i+1) (:83-84).  Each is computed once per cell and shared
-//    through LDS instead of being recomputed by three neighbours.
+//    fl(i) = c*wl(i+1) (:83-84).  fr and fl go through LDS once per stage;
+//    cell i evaluates dfrm and dflm at i and at i+1 (and the stage-1 flux at
+//    i and i-1) itself from the shared values, with the same expressions,
+//    instead of waiting for a further LDS round trip.
 //  * x / c for the pencil-constant c (C/mhdflux.c:54) is x*r corrected by one
 //    FMA residual step with r = RN(1/c); with |x| in [2^-60, 2^60] and c in
 //    [2^-30, 2^30] that is the correctly rounded quotient (Markstein's
