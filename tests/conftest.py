@@ -40,15 +40,17 @@ def load_traj(name):
     return json.loads((GOLDEN / f"traj_{name}.json").read_text())
 
 
-@pytest.fixture(autouse=True)
-def _live_gpu_output(request):
+@pytest.hookimpl(hookwrapper=True, trylast=True)
+def pytest_runtest_call(item):
     """GPU tests print their progress as they run instead of at their end, so a
     long one (the FP64 oracle of a 256^3 run takes minutes on the host) keeps
-    writing to the terminal.  Tests that read captured output keep capture."""
-    if request.node.get_closest_marker("gpu") is None or "capfd" in request.fixturenames:
+    writing to the terminal.  The wrapper runs inside the capture plugin's,
+    which resumes capture for the call phase, and suspends it again.  Tests
+    that read captured output keep capture."""
+    if item.get_closest_marker("gpu") is None or "capfd" in getattr(item, "fixturenames", ()):
         yield
         return
-    capman = request.config.pluginmanager.getplugin("capturemanager")
+    capman = item.config.pluginmanager.getplugin("capturemanager")
     with capman.global_and_fixture_disabled():
         yield
 
