@@ -124,8 +124,7 @@ __device__ __forceinline__ void transpose_advect_tile(float *__restrict__ u_out,
     // beside each other along p, whose rows hold each other's halo pencils,
     // run at the same time on one XCD and walk q together, so those halo
     // values come from L2 (transpose13+advect 19.1 -> 18.4 ms at 1024^3; the
-    // i-tile-fastest order is kept by the advect+transpose kernel, for which
-    // this order measured slower)
+    // advect+transpose tiles walk in the same order)
     const int p0 = (int)(w % (unsigned)g.npb) * kTP;
     const int rest = (int)(w / (unsigned)g.npb);
     const int it = rest % g.nti;
